@@ -1,0 +1,189 @@
+// af_device.hpp -- device arithmetic shared by the per-function kernels and
+// the fused step kernel.  All of it is compiled with -ffp-contract=off so the
+// FP64 forms evaluate exactly the operations the reference evaluates, in the
+// reference's association order (src/kernels.cpp:58-92, 108-142, 149-256,
+// 280-290, 354-367).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace af {
+
+// types.hpp:14-15
+template <typename T>
+__host__ __device__ __forceinline__ T getmax(T a, T b) { return (a > b) ? a : b; }
+template <typename T>
+__host__ __device__ __forceinline__ T getmin(T a, T b) { return (a < b) ? a : b; }
+
+// Constants of one step's face fluxes, FP64 reference form.
+struct FluxConstsF64 {
+    double dx;      // face width and the Sf divisor
+    double fn;      // Manning n
+    double thresh;  // DepthThresh
+    double maxh;    // MaxHflow
+    double gdt;     // g*dt, evaluated first as the reference does
+    double theta;
+};
+
+// Inertial flux through one interior face, the expression of
+// src/kernels.cpp:60-89 (x faces) and :110-139 (y faces).
+// z0/h0 are the upstream-index cell (i or j), z1/h1 the downstream one;
+// q0 = q_old of the face, qup/qdown its neighbours along the flow axis,
+// qt the transverse average of q_old.
+__device__ __forceinline__ double face_flux_ref(double z0, double z1, double h0, double h1,
+                                                double q0, double qup, double qdown, double qt,
+                                                const FluxConstsF64& k) {
+    double y0 = z0 + h0, y1 = z1 + h1;
+    double dh = y0 - y1;
+    double Sf = -dh / k.dx;
+    double hflow = getmax(y0, y1) - getmax(z0, z1);
+    hflow = getmax(hflow, 0.0);
+    hflow = getmin(hflow, k.maxh);
+    double Q = 0.0;
+    if (hflow > k.thresh) {
+        double qvect = sqrt(q0 * q0 + qt * qt);
+        double num = (k.theta * q0 + 0.5 * (1.0 - k.theta) * (qup + qdown)) - (k.gdt * hflow * Sf);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        Q = num / den * k.dx;
+        if (Q * dh < 0.0) {
+            num = q0 - (k.gdt * hflow * Sf);
+            Q = num / den * k.dx;
+        }
+    }
+    return Q;
+}
+
+// theta == 1 form of face_flux_ref.  With theta = 1 the reference's
+// numerator is q0 + 0*(qup+qdown) - g*dt*hflow*Sf, equal in value to
+// q0 - g*dt*hflow*Sf, and its sign-reversal branch recomputes that same
+// value, so the result equals face_flux_ref's for every finite input.
+__device__ __forceinline__ double face_flux_ref_t1(double z0, double z1, double h0, double h1,
+                                                   double q0, double qt, const FluxConstsF64& k) {
+    double y0 = z0 + h0, y1 = z1 + h1;
+    double dh = y0 - y1;
+    double hflow = getmax(y0, y1) - getmax(z0, z1);
+    hflow = getmax(hflow, 0.0);
+    hflow = getmin(hflow, k.maxh);
+    double Q = 0.0;
+    if (hflow > k.thresh) {
+        double Sf = -dh / k.dx;
+        double qvect = sqrt(q0 * q0 + qt * qt);
+        double num = (k.theta * q0) - (k.gdt * hflow * Sf);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        Q = num / den * k.dx;
+    }
+    return Q;
+}
+
+// Constants of one step's face fluxes, FP32 form.
+struct FluxConstsF32 {
+    float dx;
+    float thresh;
+    float maxh;
+    float gdt_dx;   // g*dt/dx: num = q0 + (g*dt/dx)*hflow*dh
+    float gdt_n2;   // g*dt*n^2: den = 1 + g*dt*n^2*|q|*hflow^(-7/3)
+};
+
+// FP32 form of face_flux_ref_t1.  Same physics, rearranged for the VALU:
+//   g*dt*hflow*Sf          = -(g*dt/dx)*hflow*dh
+//   g*dt*hflow*n^2*|q| / hflow^(10/3) = g*dt*n^2*|q| * hflow^(-7/3)
+// with hflow^(-7/3) = exp2(-7/3*log2(hflow)) on the hardware transcendental
+// units (hflow > DepthThresh > 0 here) and one reciprocal for num/den.
+__device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, float h1, float q0,
+                                               float qt, const FluxConstsF32& k) {
+    float y0 = z0 + h0, y1 = z1 + h1;
+    float dh = y0 - y1;
+    float hflow = getmax(y0, y1) - getmax(z0, z1);
+    hflow = getmax(hflow, 0.0f);
+    hflow = getmin(hflow, k.maxh);
+    float Q = 0.0f;
+    if (hflow > k.thresh) {
+        float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
+        float num = q0 + k.gdt_dx * hflow * dh;
+        float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
+        float den = 1.0f + k.gdt_n2 * qvect * p;
+        Q = num * k.dx * __builtin_amdgcn_rcpf(den);
+    }
+    return Q;
+}
+
+// Edge fluxes of src/kernels.cpp:154-253 from the pre-update depth.
+// West (kernels.cpp:155-179): needs the inner neighbour (z1, h1) unchecked.
+template <typename T>
+__device__ __forceinline__ T edge_west(T z0, T h0, T z1, T h1, bool nodata0, T dx, T g) {
+    if (nodata0 || h0 < T(0.001)) return T(0);
+    T y0 = z0 + h0, y1 = z1 + h1;
+    if (y0 > z0) {
+        T Sf = (y1 - y0) / dx;
+        if (Sf < 0) return -h0 * sqrt(g * h0) * dx * T(0.5);
+        return T(0);
+    }
+    return T(0);
+}
+// East and South (kernels.cpp:188-202, 234-248): outflow +h0*sqrt(g*h0)*dx/2.
+template <typename T>
+__device__ __forceinline__ T edge_out_pos(T h0, bool nodata0, T dx, T g) {
+    if (nodata0 || h0 < T(0.001)) return T(0);
+    if (h0 > T(0.001)) return h0 * sqrt(g * h0) * dx * T(0.5);
+    return T(0);
+}
+// North (kernels.cpp:211-225): outflow -h0*sqrt(g*h0)*dx/2.
+template <typename T>
+__device__ __forceinline__ T edge_out_neg(T h0, bool nodata0, T dx, T g) {
+    if (nodata0 || h0 < T(0.001)) return T(0);
+    if (h0 > T(0.001)) return -h0 * sqrt(g * h0) * dx * T(0.5);
+    return T(0);
+}
+
+// Order-preserving maps from IEEE values to unsigned keys, so that an
+// unsigned atomicMax computes a floating-point max (NaN excluded by callers).
+__host__ __device__ __forceinline__ unsigned long long fkey(double x) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &x, 8);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double fkey_inv_d(unsigned long long k) {
+    unsigned long long b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+__host__ __device__ __forceinline__ unsigned int fkey(float x) {
+    unsigned int b;
+    __builtin_memcpy(&b, &x, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float fkey_inv_f(unsigned int k) {
+    unsigned int b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+}
+
+// Linear interpolation of src/io.cpp:299-311: clamped at both ends, bracket
+// [t_i, t_{i+1}), empty series -> empty_value.
+__host__ __device__ __forceinline__ double interp_series(const double* times,
+                                                         const double* values, int n,
+                                                         double empty_value, double t) {
+    if (n <= 0) return empty_value;
+    if (t <= times[0]) return values[0];
+    if (t >= times[n - 1]) return values[n - 1];
+    for (int i = 0; i < n - 1; i++) {
+        if (t >= times[i] && t < times[i + 1]) {
+            double ratio = (t - times[i]) / (times[i + 1] - times[i]);
+            return values[i] + ratio * (values[i + 1] - values[i]);
+        }
+    }
+    return values[n - 1];
+}
+
+// dt of src/kernels.cpp:36-43
+__host__ __device__ __forceinline__ double cfl_dt(double hmax, double thresh, double cfl,
+                                                  double dx, double g, double init_tstep) {
+    double dt;
+    if (hmax > thresh) dt = cfl * dx / sqrt(g * hmax);
+    else dt = init_tstep;
+    return dt;
+}
+
+}  // namespace af

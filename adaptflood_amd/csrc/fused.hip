@@ -1,0 +1,855 @@
+// fused.hip -- host runtime of the fused step: arena layout, point plans,
+// uploads, eager and graph-replayed stepping, read-back, and the C ABI.
+#include <float.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "af_host.hpp"
+#include "fused_kernel.hpp"
+
+namespace af {
+namespace {
+
+constexpr float kNodataF32 = -1.0e30f;  // z sentinel of NODATA cells at FP32
+
+inline unsigned long long host_key(double x) {
+    unsigned long long b;
+    memcpy(&b, &x, 8);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Bump layout of one arena: offsets first, pointers after the arena is known.
+struct Layout {
+    size_t top = 0;
+    size_t take(size_t bytes) {
+        size_t o = align_up(top, 256);
+        top = o + align_up(bytes ? bytes : 1, 256);
+        return o;
+    }
+};
+
+struct HostScn {
+    int nx = 0, ny = 0;
+    long long P = 0;
+    int nstrips = 0, nsegs = 0, ntiles = 0, tile_base = 0;
+    double zmin = 0.0;
+    // offsets in the arena
+    size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
+    size_t o_src_begin, o_src_j, o_src_i, o_src_pt;
+    size_t o_ovr_begin, o_ovr_j, o_ovr_i, o_ovr_pt;
+    size_t o_pt_type, o_pt_value, o_pt_off, o_pt_len, o_series_t, o_series_v;
+    size_t o_rain_t, o_rain_r, o_ctl, o_part_vol, o_part_comp, o_part_area, o_dt_log, o_reports;
+    // host-built plan contents
+    std::vector<int> src_begin, src_j, src_i, src_pt, ovr_begin, ovr_j, ovr_i, ovr_pt;
+    std::vector<int> pt_type, pt_off, pt_len;
+    std::vector<double> pt_value;
+    int n_series = 0, n_rain = 0;
+    ScnDev dev{};
+};
+
+}  // namespace
+}  // namespace af
+
+struct af_domain {
+    int precision = AF_FP32;
+    int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36;
+    long long dt_log_cap = 0;
+    int report_cap = 64;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<af::HostScn> scn;
+    size_t o_scns = 0, o_tiles = 0, o_volbuf = 0;
+    af::ScnDev* d_scns = nullptr;
+    af::TileDesc* d_tiles = nullptr;
+    double* d_volbuf = nullptr;
+    int ntiles = 0;
+    long long k = 0;  // index of the next step to launch
+    double last_tsum = -1.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+namespace af {
+namespace {
+
+constexpr int kVolBlocks = 1024;
+
+int elem_size(int precision) { return precision == AF_FP64 ? 8 : 4; }
+
+int validate(const af_scenario_desc* d) {
+    AF_REQUIRE(d, AF_E_INVALID, "null scenario");
+    AF_REQUIRE(d->nx >= 2 && d->ny >= 2, AF_E_INVALID, "grid %d x %d: need at least 2 x 2", d->nx,
+               d->ny);
+    AF_REQUIRE((long long)d->nx * d->ny < (1LL << 31), AF_E_INVALID,
+               "grid %d x %d: more than 2^31 cells", d->nx, d->ny);
+    AF_REQUIRE(d->dem, AF_E_INVALID, "null DEM");
+    AF_REQUIRE(d->dx > 0.0, AF_E_INVALID, "cell size must be positive");
+    AF_REQUIRE(d->theta == 1.0, AF_E_INVALID,
+               "the fused step implements theta = 1 (main.cpp:124); got %g", d->theta);
+    AF_REQUIRE(d->n_points >= 0, AF_E_INVALID, "negative point count");
+    if (d->n_points > 0)
+        AF_REQUIRE(d->pt_x && d->pt_y && d->pt_type && d->pt_value, AF_E_INVALID,
+                   "null point arrays");
+    for (int p = 0; p < d->n_points; p++) {
+        long long idx = (long long)d->pt_x[p] + (long long)d->pt_y[p] * d->nx;  // io.cpp:187
+        AF_REQUIRE(idx >= 0 && idx < (long long)d->nx * d->ny, AF_E_INVALID,
+                   "point %d at grid (%d, %d) lies outside the %d x %d grid", p, d->pt_x[p],
+                   d->pt_y[p], d->nx, d->ny);
+        int ty = d->pt_type[p];
+        AF_REQUIRE(ty >= AF_BC_CLOSED && ty <= AF_BC_QVAR, AF_E_INVALID, "point %d: type %d", p,
+                   ty);
+        int len = d->pt_series_len ? d->pt_series_len[p] : 0;
+        int off = d->pt_series_off ? d->pt_series_off[p] : 0;
+        AF_REQUIRE(len >= 0 && off >= 0 && (long long)off + len <= d->n_series, AF_E_INVALID,
+                   "point %d: series [%d, %d) outside %d entries", p, off, off + len,
+                   d->n_series);
+        if (len > 0) AF_REQUIRE(d->series_t && d->series_v, AF_E_INVALID, "null series arrays");
+    }
+    if (d->rain_enabled) {
+        AF_REQUIRE(d->n_rain > 0 && d->rain_t && d->rain_r, AF_E_INVALID,
+                   "rain enabled without a series");
+    }
+    return AF_OK;
+}
+
+void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
+    h.nx = d->nx;
+    h.ny = d->ny;
+    h.P = (long long)align_up((size_t)d->nx + 1, 64);
+    h.nstrips = (d->nx + D->W - 1) / D->W;
+    h.nsegs = (d->ny + D->R - 1) / D->R;
+    h.ntiles = h.nstrips * h.nsegs;
+}
+
+void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layout& L) {
+    const size_t es = elem_size(D->precision);
+    const size_t cells = (size_t)h.P * h.ny, faces_y = (size_t)h.P * (h.ny + 1);
+    h.o_z = L.take(cells * es);
+    for (int b = 0; b < 2; b++) {
+        h.o_H[b] = L.take(cells * es);
+        h.o_Qx[b] = L.take(cells * es);
+        h.o_Qy[b] = L.take(faces_y * es);
+    }
+    int n_src = 0, n_ovr = 0;
+    for (int p = 0; p < d->n_points; p++) {
+        int ty = d->pt_type[p];
+        if (ty == AF_BC_QFIX || ty == AF_BC_QVAR) n_src++;
+        else if (ty == AF_BC_FREE || ty == AF_BC_HFIX || ty == AF_BC_HVAR) n_ovr++;
+    }
+    const size_t nt1 = (size_t)h.ntiles + 1;
+    h.o_src_begin = L.take(nt1 * 4);
+    h.o_src_j = L.take(n_src * 4);
+    h.o_src_i = L.take(n_src * 4);
+    h.o_src_pt = L.take(n_src * 4);
+    h.o_ovr_begin = L.take(nt1 * 4);
+    h.o_ovr_j = L.take(n_ovr * 4);
+    h.o_ovr_i = L.take(n_ovr * 4);
+    h.o_ovr_pt = L.take(n_ovr * 4);
+    const size_t np = (size_t)d->n_points;
+    h.o_pt_type = L.take(np * 4);
+    h.o_pt_value = L.take(np * 8);
+    h.o_pt_off = L.take(np * 4);
+    h.o_pt_len = L.take(np * 4);
+    h.o_series_t = L.take((size_t)d->n_series * 8);
+    h.o_series_v = L.take((size_t)d->n_series * 8);
+    h.o_rain_t = L.take((size_t)(d->rain_enabled ? d->n_rain : 0) * 8);
+    h.o_rain_r = L.take((size_t)(d->rain_enabled ? d->n_rain : 0) * 8);
+    h.o_ctl = L.take(sizeof(Ctl));
+    h.o_part_vol = L.take(2 * (size_t)h.ntiles * 8);
+    h.o_part_comp = L.take(2 * (size_t)h.ntiles * 8);
+    h.o_part_area = L.take(2 * (size_t)h.ntiles * 8);
+    h.o_dt_log = L.take((size_t)D->dt_log_cap * 8);
+    h.o_reports = L.take((size_t)D->report_cap * 32);
+}
+
+// Points sorted per tile by (row, column, file order): a stable sort keeps
+// file order among points of one cell, which is the order the reference's
+// sequential loops would apply them in.
+void plan_points(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
+    struct Item { int tile, j, i, pt; };
+    std::vector<Item> src, ovr;
+    for (int p = 0; p < d->n_points; p++) {
+        long long idx = (long long)d->pt_x[p] + (long long)d->pt_y[p] * d->nx;
+        int j = (int)(idx / d->nx), i = (int)(idx % d->nx);
+        int tile = (i / D->W) + (j / D->R) * h.nstrips;
+        int ty = d->pt_type[p];
+        if (ty == AF_BC_QFIX || ty == AF_BC_QVAR) src.push_back({tile, j, i, p});
+        else if (ty == AF_BC_FREE || ty == AF_BC_HFIX || ty == AF_BC_HVAR) ovr.push_back({tile, j, i, p});
+    }
+    auto key_less = [](const Item& a, const Item& b) {
+        if (a.tile != b.tile) return a.tile < b.tile;
+        if (a.j != b.j) return a.j < b.j;
+        return a.i < b.i;
+    };
+    std::stable_sort(src.begin(), src.end(), key_less);
+    std::stable_sort(ovr.begin(), ovr.end(), key_less);
+    auto build = [&](const std::vector<Item>& v, std::vector<int>& begin, std::vector<int>& J,
+                     std::vector<int>& I, std::vector<int>& PT) {
+        begin.assign(h.ntiles + 1, 0);
+        for (const Item& it : v) begin[it.tile + 1]++;
+        for (int t = 0; t < h.ntiles; t++) begin[t + 1] += begin[t];
+        J.clear(); I.clear(); PT.clear();
+        for (const Item& it : v) { J.push_back(it.j); I.push_back(it.i); PT.push_back(it.pt); }
+    };
+    build(src, h.src_begin, h.src_j, h.src_i, h.src_pt);
+    build(ovr, h.ovr_begin, h.ovr_j, h.ovr_i, h.ovr_pt);
+    h.pt_type.assign(d->pt_type, d->pt_type + d->n_points);
+    h.pt_value.assign(d->pt_value, d->pt_value + d->n_points);
+    h.pt_off.assign(d->n_points, 0);
+    h.pt_len.assign(d->n_points, 0);
+    for (int p = 0; p < d->n_points; p++) {
+        if (d->pt_series_off) h.pt_off[p] = d->pt_series_off[p];
+        if (d->pt_series_len) h.pt_len[p] = d->pt_series_len[p];
+    }
+    h.n_series = d->n_series;
+    h.n_rain = d->rain_enabled ? d->n_rain : 0;
+}
+
+int configure(af_domain* D, int precision, const af_tuning* tun) {
+    AF_REQUIRE(precision == AF_FP32 || precision == AF_FP64, AF_E_INVALID, "precision %d",
+               precision);
+    D->precision = precision;
+    D->V = precision == AF_FP64 ? 2 : 4;
+    D->W = 64 * D->V;
+    D->R = 128;
+    D->wpb = 4;
+    D->graph_steps = 36;
+    D->dt_log_cap = 0;
+    D->report_cap = 256;
+    if (tun) {
+        if (tun->tile_rows > 0) D->R = tun->tile_rows;
+        if (tun->waves_per_block > 0) D->wpb = tun->waves_per_block;
+        if (tun->graph_steps > 0) D->graph_steps = tun->graph_steps;
+        if (tun->dt_log_cap > 0) D->dt_log_cap = tun->dt_log_cap;
+        if (tun->report_cap > 0) D->report_cap = tun->report_cap;
+    }
+    AF_REQUIRE(D->R >= 1 && D->R <= 65536, AF_E_INVALID, "tile_rows %d", D->R);
+    AF_REQUIRE(D->wpb == 1 || D->wpb == 2 || D->wpb == 4, AF_E_INVALID, "waves_per_block %d",
+               D->wpb);
+    AF_REQUIRE(D->graph_steps % 6 == 0 && D->graph_steps <= 6000, AF_E_INVALID,
+               "graph_steps %d must be a positive multiple of 6", D->graph_steps);
+    return AF_OK;
+}
+
+size_t plan_all(af_domain* D, const af_scenario_desc* descs, int n) {
+    Layout L;
+    D->scn.assign(n, HostScn{});
+    int tiles = 0;
+    for (int s = 0; s < n; s++) {
+        plan_geometry(D->scn[s], &descs[s], D);
+        D->scn[s].tile_base = tiles;
+        tiles += D->scn[s].ntiles;
+    }
+    D->ntiles = tiles;
+    D->o_scns = L.take(sizeof(ScnDev) * (size_t)n);
+    D->o_tiles = L.take(sizeof(TileDesc) * (size_t)tiles);
+    D->o_volbuf = L.take(sizeof(double) * 3 * kVolBlocks);
+    for (int s = 0; s < n; s++) plan_layout(D->scn[s], &descs[s], D, L);
+    return L.top;
+}
+
+template <typename T>
+void convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>& z,
+                    std::vector<T>& H, double& zmin, double& hmax) {
+    const long long P = h.P;
+    const bool f64 = sizeof(T) == 8;
+    zmin = 0.0;
+    if (!f64) {
+        bool any = false;
+        double m = 0.0;
+        for (long long c = 0; c < (long long)d->nx * d->ny; c++) {
+            double v = d->dem[c];
+            if (v == d->nodata) continue;
+            if (!any || v < m) m = v;
+            any = true;
+        }
+        zmin = any ? m : 0.0;
+    }
+    z.assign((size_t)P * d->ny, T(0));
+    H.assign((size_t)P * d->ny, T(0));
+    hmax = -DBL_MAX;
+    for (int j = 0; j < d->ny; j++) {
+        for (int i = 0; i < d->nx; i++) {
+            const long long c = i + (long long)j * d->nx;
+            const double v = d->dem[c];
+            const bool nod = (v == d->nodata);
+            T zt;
+            if (f64) zt = (T)v;
+            else zt = nod ? (T)kNodataF32 : (T)(v - zmin);
+            z[(size_t)j * P + i] = zt;
+            T ht = d->h0 ? (T)d->h0[c] : T(0);
+            H[(size_t)j * P + i] = ht;
+            if (!nod && (double)ht > hmax) hmax = (double)ht;  // kernels.cpp:32
+        }
+    }
+}
+
+int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t st,
+                    int parity, long long kmod) {
+    HostScn& h = D->scn[s];
+    plan_points(h, d, D);
+    char* A = D->arena;
+    // device pointers
+    ScnDev& v = h.dev;
+    v = ScnDev{};
+    v.nx = h.nx;
+    v.ny = h.ny;
+    v.P = h.P;
+    v.nstrips = h.nstrips;
+    v.tile_rows = D->R;
+    v.tile_base = h.tile_base;
+    v.ntiles = h.ntiles;
+    v.z = A + h.o_z;
+    for (int b = 0; b < 2; b++) {
+        v.H[b] = A + h.o_H[b];
+        v.Qx[b] = A + h.o_Qx[b];
+        v.Qy[b] = A + h.o_Qy[b];
+    }
+    v.dx = d->dx;
+    v.dxinv = 1.0 / d->dx;
+    v.dA = d->dx * d->dx;
+    v.fn = d->fpn;
+    v.thresh = d->depth_thresh;
+    v.maxh = d->max_hflow;
+    v.init_tstep = d->init_tstep;
+    v.cfl = d->cfl;
+    v.g = d->g;
+    v.theta = d->theta;
+    v.sim_time = d->sim_time;
+    v.mass_int = d->mass_int;
+    v.nodata_t = D->precision == AF_FP64 ? d->nodata : (double)kNodataF32;
+    v.west_free = d->west_free ? 1 : 0;
+    v.east_free = d->east_free ? 1 : 0;
+    v.north_free = d->north_free ? 1 : 0;
+    v.south_free = d->south_free ? 1 : 0;
+    v.src_begin = reinterpret_cast<const int*>(A + h.o_src_begin);
+    v.src_j = reinterpret_cast<const int*>(A + h.o_src_j);
+    v.src_i = reinterpret_cast<const int*>(A + h.o_src_i);
+    v.src_pt = reinterpret_cast<const int*>(A + h.o_src_pt);
+    v.ovr_begin = reinterpret_cast<const int*>(A + h.o_ovr_begin);
+    v.ovr_j = reinterpret_cast<const int*>(A + h.o_ovr_j);
+    v.ovr_i = reinterpret_cast<const int*>(A + h.o_ovr_i);
+    v.ovr_pt = reinterpret_cast<const int*>(A + h.o_ovr_pt);
+    v.pt_type = reinterpret_cast<const int*>(A + h.o_pt_type);
+    v.pt_value = reinterpret_cast<const double*>(A + h.o_pt_value);
+    v.pt_off = reinterpret_cast<const int*>(A + h.o_pt_off);
+    v.pt_len = reinterpret_cast<const int*>(A + h.o_pt_len);
+    v.series_t = reinterpret_cast<const double*>(A + h.o_series_t);
+    v.series_v = reinterpret_cast<const double*>(A + h.o_series_v);
+    v.rain_enabled = d->rain_enabled ? 1 : 0;
+    v.n_rain = h.n_rain;
+    v.rain_t = reinterpret_cast<const double*>(A + h.o_rain_t);
+    v.rain_r = reinterpret_cast<const double*>(A + h.o_rain_r);
+    v.ctl = reinterpret_cast<Ctl*>(A + h.o_ctl);
+    v.part_vol = reinterpret_cast<double*>(A + h.o_part_vol);
+    v.part_comp = reinterpret_cast<double*>(A + h.o_part_comp);
+    v.part_area = reinterpret_cast<unsigned long long*>(A + h.o_part_area);
+    v.dt_log = reinterpret_cast<double*>(A + h.o_dt_log);
+    v.dt_log_cap = D->dt_log_cap;
+    v.reports = reinterpret_cast<double*>(A + h.o_reports);
+    v.report_cap = D->report_cap;
+
+    // state: zero everything, then z and H0 into buffer `parity`
+    const size_t es = elem_size(D->precision);
+    const size_t cells = (size_t)h.P * h.ny, faces_y = (size_t)h.P * (h.ny + 1);
+    AF_HIP(hipMemsetAsync(A + h.o_z, 0, cells * es, st));
+    for (int b = 0; b < 2; b++) {
+        AF_HIP(hipMemsetAsync(A + h.o_H[b], 0, cells * es, st));
+        AF_HIP(hipMemsetAsync(A + h.o_Qx[b], 0, cells * es, st));
+        AF_HIP(hipMemsetAsync(A + h.o_Qy[b], 0, faces_y * es, st));
+    }
+    double hmax = -DBL_MAX;
+    if (D->precision == AF_FP64) {
+        std::vector<double> z, H;
+        convert_inputs<double>(d, h, z, H, h.zmin, hmax);
+        AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
+        AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
+        AF_HIP(hipStreamSynchronize(st));
+    } else {
+        std::vector<float> z, H;
+        convert_inputs<float>(d, h, z, H, h.zmin, hmax);
+        AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
+        AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
+        AF_HIP(hipStreamSynchronize(st));
+    }
+    auto up = [&](size_t off, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return AF_OK;
+        AF_HIP(hipMemcpyAsync(A + off, src, bytes, hipMemcpyHostToDevice, st));
+        return AF_OK;
+    };
+    int rc = AF_OK;
+    if ((rc = up(h.o_src_begin, h.src_begin.data(), h.src_begin.size() * 4))) return rc;
+    if ((rc = up(h.o_src_j, h.src_j.data(), h.src_j.size() * 4))) return rc;
+    if ((rc = up(h.o_src_i, h.src_i.data(), h.src_i.size() * 4))) return rc;
+    if ((rc = up(h.o_src_pt, h.src_pt.data(), h.src_pt.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_begin, h.ovr_begin.data(), h.ovr_begin.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_j, h.ovr_j.data(), h.ovr_j.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_i, h.ovr_i.data(), h.ovr_i.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_pt, h.ovr_pt.data(), h.ovr_pt.size() * 4))) return rc;
+    if ((rc = up(h.o_pt_type, h.pt_type.data(), h.pt_type.size() * 4))) return rc;
+    if ((rc = up(h.o_pt_value, h.pt_value.data(), h.pt_value.size() * 8))) return rc;
+    if ((rc = up(h.o_pt_off, h.pt_off.data(), h.pt_off.size() * 4))) return rc;
+    if ((rc = up(h.o_pt_len, h.pt_len.data(), h.pt_len.size() * 4))) return rc;
+    if ((rc = up(h.o_series_t, d->series_t, (size_t)h.n_series * 8))) return rc;
+    if ((rc = up(h.o_series_v, d->series_v, (size_t)h.n_series * 8))) return rc;
+    if ((rc = up(h.o_rain_t, d->rain_t, (size_t)h.n_rain * 8))) return rc;
+    if ((rc = up(h.o_rain_r, d->rain_r, (size_t)h.n_rain * 8))) return rc;
+
+    Ctl c{};
+    memset(&c, 0, sizeof(c));
+    const int cur = parity;
+    c.t[cur] = 0.0;                       // main.cpp:197
+    c.next_mass[cur] = d->mass_int;       // main.cpp:200
+    const int hs = (int)(kmod % 3);
+    c.hmax[hs] = host_key(hmax);
+    c.hmax[(hs + 1) % 3] = host_key(-DBL_MAX);
+    c.hmax[(hs + 2) % 3] = host_key(-DBL_MAX);
+    c.t[cur ^ 1] = 0.0;
+    c.next_mass[cur ^ 1] = d->mass_int;
+    if ((rc = up(h.o_ctl, &c, sizeof(c)))) return rc;
+    AF_HIP(hipStreamSynchronize(st));
+    return AF_OK;
+}
+
+template <typename T, int V, bool F64>
+int launch_step_t(af_domain* D, int kmod6, hipStream_t st) {
+    const int wpb = D->wpb;
+    const unsigned blocks = (unsigned)((D->ntiles + wpb - 1) / wpb);
+    if (wpb == 4)
+        hipLaunchKernelGGL((k_step<T, V, F64, 4>), dim3(blocks), dim3(256), 0, st, D->d_scns,
+                           D->d_tiles, D->ntiles, kmod6);
+    else if (wpb == 2)
+        hipLaunchKernelGGL((k_step<T, V, F64, 2>), dim3(blocks), dim3(128), 0, st, D->d_scns,
+                           D->d_tiles, D->ntiles, kmod6);
+    else
+        hipLaunchKernelGGL((k_step<T, V, F64, 1>), dim3(blocks), dim3(64), 0, st, D->d_scns,
+                           D->d_tiles, D->ntiles, kmod6);
+    AF_HIP(hipGetLastError());
+    return AF_OK;
+}
+
+int launch_step(af_domain* D, int kmod6, hipStream_t st) {
+    if (D->precision == AF_FP64) return launch_step_t<double, 2, true>(D, kmod6, st);
+    return launch_step_t<float, 4, false>(D, kmod6, st);
+}
+
+int build_graph(af_domain* D, hipStream_t st) {
+    if (D->gexec) return AF_OK;
+    hipStream_t cs;
+    AF_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(cs);
+        set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+        return AF_E_HIP;
+    }
+    int rc = AF_OK;
+    for (int i = 0; i < D->graph_steps && rc == AF_OK; i++) rc = launch_step(D, i % 6, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc != AF_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return AF_E_HIP;
+    }
+    D->graph = g;
+    AF_HIP(hipGraphInstantiate(&D->gexec, g, nullptr, nullptr, 0));
+    (void)st;
+    return AF_OK;
+}
+
+// Reduction of sum(H*dA) and wet area for af_domain_volume.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_volume(const T* __restrict__ H, long long P, int nx, int ny, double dA, double* __restrict__ out) {
+    double s = 0.0, c = 0.0;
+    double wet = 0.0;
+    const long long n = (long long)nx * ny;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < n; t += (long long)gridDim.x * 256) {
+        const long long j = t / nx, i = t % nx;
+        const double h = (double)H[j * P + i];
+        kahan_add(s, c, h * dA);
+        if (h > 0.01) wet += 1.0;
+    }
+    __shared__ double ss[256], sc2[256], sw[256];
+    ss[threadIdx.x] = s;
+    sc2[threadIdx.x] = c;
+    sw[threadIdx.x] = wet;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0, w = 0.0;
+        for (int k = 0; k < 256; k++) {
+            kahan_add(a, b, ss[k]);
+            b += sc2[k];
+            w += sw[k];
+        }
+        out[3 * blockIdx.x + 0] = a;
+        out[3 * blockIdx.x + 1] = b;
+        out[3 * blockIdx.x + 2] = w;
+    }
+}
+
+int current_parity(af_domain* D, int s, long long* n_done, hipStream_t st) {
+    Ctl c;
+    AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    if (n_done) *n_done = c.n_done;
+    return (int)(c.n_done & 1);
+}
+
+__global__ void k_finalize_reports(const ScnDev* __restrict__ scns, int n) {
+    const int lane = threadIdx.x & 63;
+    for (int s = 0; s < n; s++) Stepper<double, 2, true>::finalize_report(scns[s], lane);
+}
+
+int finalize_reports(af_domain* D, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize_reports, dim3(1), dim3(64), 0, st, D->d_scns,
+                       (int)D->scn.size());
+    AF_HIP(hipGetLastError());
+    return AF_OK;
+}
+
+}  // namespace
+}  // namespace af
+
+using namespace af;
+
+extern "C" {
+
+int af_batch_arena_bytes(const af_scenario_desc* descs, int n, int precision,
+                         const af_tuning* tuning, size_t* bytes) {
+    AF_REQUIRE(descs && n >= 1 && bytes, AF_E_INVALID, "arena_bytes: bad arguments");
+    af_domain tmp;
+    int rc = configure(&tmp, precision, tuning);
+    if (rc) return rc;
+    for (int s = 0; s < n; s++) {
+        rc = validate(&descs[s]);
+        if (rc) return rc;
+    }
+    *bytes = plan_all(&tmp, descs, n);
+    return AF_OK;
+}
+
+int af_domain_arena_bytes(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                          size_t* bytes) {
+    return af_batch_arena_bytes(desc, 1, precision, tuning, bytes);
+}
+
+int af_batch_create(const af_scenario_desc* descs, int n, int precision, const af_tuning* tuning,
+                    void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out) {
+    AF_REQUIRE(descs && n >= 1 && arena && out, AF_E_INVALID, "create: bad arguments");
+    AF_REQUIRE(((uintptr_t)arena & 255) == 0, AF_E_INVALID, "arena must be 256-byte aligned");
+    af_domain* D = new af_domain();
+    int rc = configure(D, precision, tuning);
+    for (int s = 0; s < n && rc == AF_OK; s++) rc = validate(&descs[s]);
+    if (rc) { delete D; return rc; }
+    size_t need = plan_all(D, descs, n);
+    if (need > arena_bytes) {
+        delete D;
+        set_error("arena too small: need %zu bytes, got %zu", need, arena_bytes);
+        return AF_E_CAPACITY;
+    }
+    D->arena = reinterpret_cast<char*>(arena);
+    D->arena_bytes = arena_bytes;
+    D->d_scns = reinterpret_cast<ScnDev*>(D->arena + D->o_scns);
+    D->d_tiles = reinterpret_cast<TileDesc*>(D->arena + D->o_tiles);
+    D->d_volbuf = reinterpret_cast<double*>(D->arena + D->o_volbuf);
+    hipStream_t st = to_stream(stream);
+    std::vector<ScnDev> devs(n);
+    std::vector<TileDesc> tiles;
+    tiles.reserve(D->ntiles);
+    for (int s = 0; s < n && rc == AF_OK; s++) {
+        rc = upload_scenario(D, s, &descs[s], st, 0, 0);
+        if (rc) break;
+        devs[s] = D->scn[s].dev;
+        const HostScn& h = D->scn[s];
+        for (int seg = 0; seg < h.nsegs; seg++)
+            for (int strip = 0; strip < h.nstrips; strip++)
+                tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
+    }
+    if (rc == AF_OK) {
+        hipError_t e = hipMemcpyAsync(D->d_scns, devs.data(), sizeof(ScnDev) * n,
+                                      hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(D->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
+                               hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("upload of tables: %s", hipGetErrorString(e));
+            rc = AF_E_HIP;
+        }
+    }
+    if (rc) { delete D; return rc; }
+    *out = D;
+    return AF_OK;
+}
+
+int af_domain_create(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                     void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out) {
+    return af_batch_create(desc, 1, precision, tuning, arena, arena_bytes, stream, out);
+}
+
+int af_domain_destroy(af_domain* D) {
+    if (!D) return AF_OK;
+    if (D->gexec) (void)hipGraphExecDestroy(D->gexec);
+    if (D->graph) (void)hipGraphDestroy(D->graph);
+    delete D;
+    return AF_OK;
+}
+
+int af_domain_advance(af_domain* D, int64_t max_steps, int use_graph, af_stream_t stream) {
+    AF_REQUIRE(D && max_steps >= 0, AF_E_INVALID, "advance: bad arguments");
+    hipStream_t st = to_stream(stream);
+    if (use_graph) {
+        int rc = build_graph(D, st);
+        if (rc) return rc;
+    }
+    int64_t left = max_steps;
+    while (left > 0) {
+        if (use_graph && (D->k % 6) == 0 && left >= D->graph_steps) {
+            AF_HIP(hipGraphLaunch(D->gexec, st));
+            D->k += D->graph_steps;
+            left -= D->graph_steps;
+        } else {
+            int rc = launch_step(D, (int)(D->k % 6), st);
+            if (rc) return rc;
+            D->k++;
+            left--;
+        }
+    }
+    return AF_OK;
+}
+
+int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
+    AF_REQUIRE(D && all_done, AF_E_INVALID, "finished: bad arguments");
+    hipStream_t st = to_stream(stream);
+    const int n = (int)D->scn.size();
+    std::vector<Ctl> c(n);
+    for (int s = 0; s < n; s++)
+        AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    int done = 1;
+    for (int s = 0; s < n; s++) {
+        const int par = (int)(D->k & 1);
+        if (c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
+    }
+    *all_done = done;
+    return AF_OK;
+}
+
+int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t stream) {
+    AF_REQUIRE(D && check_every > 0, AF_E_INVALID, "run: bad arguments");
+    hipStream_t st = to_stream(stream);
+    const int n = (int)D->scn.size();
+    std::vector<Ctl> c(n);
+    long long prev_done = -1;
+    for (;;) {
+        for (int s = 0; s < n; s++)
+            AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        AF_HIP(hipStreamSynchronize(st));
+        const int par = (int)(D->k & 1);
+        int done = 1;
+        long long steps = 0;
+        double tsum = 0.0;
+        for (int s = 0; s < n; s++) {
+            if (c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
+            steps += c[s].n_done;
+            tsum += c[s].t[par];
+        }
+        if (done) break;
+        // a clock that no longer advances (dt == 0 after an overflow) would never end
+        if (prev_done >= 0 && steps > prev_done && !(tsum > D->last_tsum)) {
+            set_error("simulation clock stalled at t = %g", tsum);
+            return AF_E_STATE;
+        }
+        prev_done = steps;
+        D->last_tsum = tsum;
+        int rc = af_domain_advance(D, check_every, use_graph, stream);
+        if (rc) return rc;
+    }
+    return AF_OK;
+}
+
+int af_batch_info(af_domain* D, int s, af_run_info* info, af_stream_t stream) {
+    AF_REQUIRE(D && info && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "info: bad arguments");
+    hipStream_t st = to_stream(stream);
+    int rc = finalize_reports(D, st);
+    if (rc) return rc;
+    Ctl c;
+    AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    const int tpar = (int)(D->k & 1);  // every launched step writes t[(k+1) & 1]
+    info->t = c.t[tpar];
+    info->steps_done = c.n_done;
+    info->last_dt = c.last_dt;
+    info->n_reports = c.n_reports;
+    info->finished = !(c.t[tpar] < D->scn[s].dev.sim_time);
+    info->hmax = fkey_inv_d(c.hmax[(int)(D->k % 3)]);
+    return AF_OK;
+}
+
+int af_domain_info(af_domain* D, af_run_info* info, af_stream_t stream) {
+    return af_batch_info(D, 0, info, stream);
+}
+
+int af_batch_read(af_domain* D, int s, int which, double* out, af_stream_t stream) {
+    AF_REQUIRE(D && out && s >= 0 && s < (int)D->scn.size() && which >= 0 && which <= 4,
+               AF_E_INVALID, "read: bad arguments");
+    hipStream_t st = to_stream(stream);
+    HostScn& h = D->scn[s];
+    long long n_done = 0;
+    const int par = current_parity(D, s, &n_done, st);
+    if (par < 0) return par;
+    const size_t es = elem_size(D->precision);
+    const int rows = (which == 2 || which == 4) ? h.ny + 1 : h.ny;
+    const void* src = (which == 0) ? h.dev.H[par]
+                      : (which == 1 || which == 3) ? h.dev.Qx[par] : h.dev.Qy[par];
+    std::vector<char> buf((size_t)h.P * rows * es);
+    AF_HIP(hipMemcpyAsync(buf.data(), src, buf.size(), hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    const int nx = h.nx, ny = h.ny;
+    auto get = [&](long long k) -> double {
+        return es == 8 ? reinterpret_cast<const double*>(buf.data())[k]
+                       : (double)reinterpret_cast<const float*>(buf.data())[k];
+    };
+    auto get_old = [&](long long k) -> double {  // q_old = Q * (1/dx) in the state precision
+        if (es == 8) return reinterpret_cast<const double*>(buf.data())[k] * h.dev.dxinv;
+        return (double)(reinterpret_cast<const float*>(buf.data())[k] * (float)h.dev.dxinv);
+    };
+    if (which == 0) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) out[i + (long long)j * nx] = get((long long)j * h.P + i);
+        return AF_OK;
+    }
+    const long long PR = nx + 1;
+    for (long long k = 0; k < PR * (ny + 1); k++) out[k] = 0.0;
+    const bool old = (which >= 3);
+    if (which == 1 || which == 3) {
+        for (int j = 0; j < ny; j++)
+            for (int f = 0; f <= nx; f++) {
+                const long long k = (long long)j * h.P + f;
+                out[f + j * PR] = old ? get_old(k) : get(k);
+            }
+    } else {
+        for (int g = 0; g <= ny; g++)
+            for (int i = 0; i < nx; i++) {
+                const long long k = (long long)g * h.P + i;
+                out[i + g * PR] = old ? get_old(k) : get(k);
+            }
+    }
+    return AF_OK;
+}
+
+int af_domain_read(af_domain* D, int which, double* out, af_stream_t stream) {
+    return af_batch_read(D, 0, which, out, stream);
+}
+
+int af_batch_read_dt_log(af_domain* D, int s, double* out, int64_t cap, int64_t* n,
+                         af_stream_t stream) {
+    AF_REQUIRE(D && n && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "dt_log: bad arguments");
+    hipStream_t st = to_stream(stream);
+    long long done = 0;
+    if (current_parity(D, s, &done, st) < 0) return AF_E_HIP;
+    long long avail = std::min<long long>(done, D->dt_log_cap);
+    long long m = std::min<long long>(avail, cap);
+    if (m > 0 && out)
+        AF_HIP(hipMemcpy(out, D->scn[s].dev.dt_log, (size_t)m * 8, hipMemcpyDeviceToHost));
+    *n = avail;
+    return AF_OK;
+}
+
+int af_domain_read_dt_log(af_domain* D, double* out, int64_t cap, int64_t* n,
+                          af_stream_t stream) {
+    return af_batch_read_dt_log(D, 0, out, cap, n, stream);
+}
+
+int af_batch_read_reports(af_domain* D, int s, double* out, int32_t cap, int32_t* n,
+                          af_stream_t stream) {
+    AF_REQUIRE(D && n && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID,
+               "reports: bad arguments");
+    hipStream_t st = to_stream(stream);
+    int rc = finalize_reports(D, st);
+    if (rc) return rc;
+    Ctl c;
+    AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    int avail = std::min(c.n_reports, D->report_cap);
+    int m = std::min(avail, cap);
+    if (m > 0 && out)
+        AF_HIP(hipMemcpy(out, D->scn[s].dev.reports, (size_t)m * 32, hipMemcpyDeviceToHost));
+    *n = c.n_reports;
+    return AF_OK;
+}
+
+int af_domain_read_reports(af_domain* D, double* out, int32_t cap, int32_t* n,
+                           af_stream_t stream) {
+    return af_batch_read_reports(D, 0, out, cap, n, stream);
+}
+
+int af_batch_volume(af_domain* D, int s, double* vol, double* area, af_stream_t stream) {
+    AF_REQUIRE(D && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "volume: bad arguments");
+    hipStream_t st = to_stream(stream);
+    HostScn& h = D->scn[s];
+    const int par = current_parity(D, s, nullptr, st);
+    if (par < 0) return par;
+    const long long n = (long long)h.nx * h.ny;
+    unsigned blocks = blocks_for(n, 256);
+    if (blocks > (unsigned)kVolBlocks) blocks = kVolBlocks;
+    if (D->precision == AF_FP64)
+        hipLaunchKernelGGL(k_volume<double>, dim3(blocks), dim3(256), 0, st,
+                           (const double*)h.dev.H[par], h.P, h.nx, h.ny, h.dev.dA, D->d_volbuf);
+    else
+        hipLaunchKernelGGL(k_volume<float>, dim3(blocks), dim3(256), 0, st,
+                           (const float*)h.dev.H[par], h.P, h.nx, h.ny, h.dev.dA, D->d_volbuf);
+    AF_HIP(hipGetLastError());
+    std::vector<double> part(3 * blocks);
+    AF_HIP(hipMemcpyAsync(part.data(), D->d_volbuf, part.size() * 8, hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    long double s_ = 0.0L, w = 0.0L;
+    for (unsigned b = 0; b < blocks; b++) {
+        s_ += (long double)part[3 * b] + (long double)part[3 * b + 1];
+        w += part[3 * b + 2];
+    }
+    if (vol) *vol = (double)s_;
+    if (area) *area = (double)(w * h.dev.dA);
+    return AF_OK;
+}
+
+int af_domain_volume(af_domain* D, double* vol, double* area, af_stream_t stream) {
+    return af_batch_volume(D, 0, vol, area, stream);
+}
+
+int af_domain_state_ptr(af_domain* D, int which, void** ptr, int64_t* pitch_elems) {
+    AF_REQUIRE(D && ptr && which >= 0 && which <= 2, AF_E_INVALID, "state_ptr: bad arguments");
+    long long done = 0;
+    int par = current_parity(D, 0, &done, nullptr);
+    if (par < 0) return par;
+    HostScn& h = D->scn[0];
+    *ptr = which == 0 ? h.dev.H[par] : which == 1 ? h.dev.Qx[par] : h.dev.Qy[par];
+    if (pitch_elems) *pitch_elems = h.P;
+    return AF_OK;
+}
+
+int af_batch_size(af_domain* D, int* n_scenarios, int* n_tiles) {
+    AF_REQUIRE(D, AF_E_INVALID, "batch_size: null");
+    if (n_scenarios) *n_scenarios = (int)D->scn.size();
+    if (n_tiles) *n_tiles = D->ntiles;
+    return AF_OK;
+}
+
+int af_batch_launch_step(af_domain* D, af_stream_t stream) {
+    AF_REQUIRE(D, AF_E_INVALID, "launch_step: null");
+    int rc = launch_step(D, (int)(D->k % 6), to_stream(stream));
+    if (rc) return rc;
+    D->k++;
+    return AF_OK;
+}
+
+}  // extern "C"

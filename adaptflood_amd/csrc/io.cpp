@@ -1,0 +1,571 @@
+// io.cpp -- readers for the reference's input formats and its ESRI-ASCII
+// writer, with the reference's parsing behaviour:
+//   .par        read_par     src/io.cpp:9-47
+//   .dem.ascii  read_dem     src/io.cpp:132-161
+//   .bci        read_bci     src/io.cpp:164-254
+//   .bdy        read_bdy     src/io.cpp:257-296
+//   rain        read_rain    src/io.cpp:103-129
+//   interpolation            src/io.cpp:299-311
+//   write_asc                src/io.cpp:314-334
+// Differences are limited to inputs on which the reference has undefined
+// behaviour (over-long tokens, a missing header number): those are rejected
+// with AF_E_IO instead.
+#include <errno.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/adaptflood.h"
+
+namespace af {
+void set_error(const char* fmt, ...);
+}
+
+namespace {
+
+struct Point {
+    int x = 0, y = 0;
+    int type = AF_BC_CLOSED;
+    double value = 0.0;
+    std::string name;
+    std::vector<double> times, values;
+};
+
+}  // namespace
+
+struct af_case {
+    af_par par;
+    bool have_dem = false;
+    int ncols = 0, nrows = 0;
+    double xll = 0, yll = 0, cellsize = 0, nodata = -9999.0;
+    std::vector<double> dem;
+    std::vector<Point> points;
+    std::vector<char> line_sides;  // FREE sides in file order
+    bool rain_enabled = false;
+    std::vector<double> rain_t, rain_r;
+    // flattened views for af_case_scenario
+    std::vector<int32_t> px, py, ptype, poff, plen;
+    std::vector<double> pvalue, st, sv;
+};
+
+namespace {
+
+void copy_str(char* dst, size_t cap, const std::string& s) {
+    size_t n = s.size() < cap - 1 ? s.size() : cap - 1;
+    memcpy(dst, s.data(), n);
+    dst[n] = 0;
+}
+
+// std::stod / std::stoi semantics: leading whitespace, longest numeric
+// prefix, failure when nothing converts.
+bool to_double(const std::string& v, double* out) {
+    const char* b = v.c_str();
+    char* e = nullptr;
+    errno = 0;
+    double d = strtod(b, &e);
+    if (e == b || errno == ERANGE) return false;
+    *out = d;
+    return true;
+}
+bool to_int(const std::string& v, int* out) {
+    const char* b = v.c_str();
+    char* e = nullptr;
+    errno = 0;
+    long l = strtol(b, &e, 10);
+    if (e == b || errno == ERANGE || l < -2147483648L || l > 2147483647L) return false;
+    *out = (int)l;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// types.hpp:18-38
+void af_par_defaults(af_par* p) {
+    memset(p, 0, sizeof(*p));
+    copy_str(p->dem_file, sizeof(p->dem_file), "dem.ascii");
+    copy_str(p->bci_file, sizeof(p->bci_file), "bci.bci");
+    copy_str(p->bdy_file, sizeof(p->bdy_file), "bdy.bdy");
+    copy_str(p->output_dir, sizeof(p->output_dir), "results");
+    p->sim_time = 3600.0;
+    p->save_int = 600.0;
+    p->mass_int = 60.0;
+    p->stage_int = 60.0;
+    p->init_tstep = 10.0;
+    p->cfl = 0.7;
+    p->fpn = 0.035;
+    p->depth_thresh = 0.001;
+    p->nodata = -9999.0;
+    p->output_format = 0;
+    p->voutput = 0;
+    p->async_output = 1;
+}
+
+int af_read_par(const char* path, af_par* p) {
+    if (!path || !p) {
+        af::set_error("read_par: null argument");
+        return AF_E_INVALID;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        af::set_error("Cannot open parameter file %s", path);
+        return AF_E_IO;
+    }
+    std::string line;
+    int lineno = 0;
+    while (std::getline(f, line)) {
+        lineno++;
+        if (line.empty() || line[0] == '#') continue;  // io.cpp:19
+        std::istringstream iss(line);
+        std::string key, value;
+        if (!(iss >> key)) continue;
+        iss >> value;  // may stay empty
+        double d = 0.0;
+        int iv = 0;
+        auto num = [&](double* dst) -> bool {
+            if (!to_double(value, &d)) {
+                af::set_error("%s:%d: key %s needs a number, got '%s'", path, lineno, key.c_str(),
+                              value.c_str());
+                return false;
+            }
+            *dst = d;
+            return true;
+        };
+        if (key == "DEMfile") copy_str(p->dem_file, sizeof(p->dem_file), value);
+        else if (key == "bcifile") copy_str(p->bci_file, sizeof(p->bci_file), value);
+        else if (key == "bdyfile") copy_str(p->bdy_file, sizeof(p->bdy_file), value);
+        else if (key == "dirroot") copy_str(p->output_dir, sizeof(p->output_dir), value);
+        else if (key == "sim_time") { if (!num(&p->sim_time)) return AF_E_IO; }
+        else if (key == "saveint") { if (!num(&p->save_int)) return AF_E_IO; }
+        else if (key == "massint") { if (!num(&p->mass_int)) return AF_E_IO; }
+        else if (key == "initial_tstep") { if (!num(&p->init_tstep)) return AF_E_IO; }
+        else if (key == "cfl") { if (!num(&p->cfl)) return AF_E_IO; }
+        else if (key == "fpfric") { if (!num(&p->fpn)) return AF_E_IO; }
+        else if (key == "depth_thresh") { if (!num(&p->depth_thresh)) return AF_E_IO; }
+        else if (key == "rainfile") copy_str(p->rain_file, sizeof(p->rain_file), value);
+        else if (key == "stagefile") copy_str(p->stage_file, sizeof(p->stage_file), value);
+        else if (key == "stageint") { if (!num(&p->stage_int)) return AF_E_IO; }
+        else if (key == "output_format") {
+            if (!to_int(value, &iv)) {
+                af::set_error("%s:%d: output_format needs an integer", path, lineno);
+                return AF_E_IO;
+            }
+            p->output_format = iv;
+        } else if (key == "voutput") p->voutput = 1;
+        else if (key == "async_output") {
+            if (!to_int(value, &iv)) {
+                af::set_error("%s:%d: async_output needs an integer", path, lineno);
+                return AF_E_IO;
+            }
+            p->async_output = (iv != 0);
+        }
+        // any other key is ignored, as in the reference
+    }
+    return AF_OK;
+}
+
+af_case* af_case_new(void) {
+    af_case* c = new af_case();
+    af_par_defaults(&c->par);
+    return c;
+}
+
+void af_case_free(af_case* c) { delete c; }
+
+// read_dem: six positional header pairs (names ignored), then nrows rows of
+// ncols values, north row first.  Values that fail to parse stay 0, as
+// fscanf leaves the zero-initialised vector untouched.
+int af_case_read_dem(af_case* c, const char* path) {
+    if (!c || !path) {
+        af::set_error("read_dem: null argument");
+        return AF_E_INVALID;
+    }
+    FILE* fp = fopen(path, "rb");
+    if (!fp) {
+        af::set_error("Cannot open DEM file %s", path);
+        return AF_E_IO;
+    }
+    std::vector<char> buf;
+    {
+        char tmp[1 << 16];
+        size_t n;
+        while ((n = fread(tmp, 1, sizeof(tmp), fp)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    }
+    fclose(fp);
+    buf.push_back(0);
+    const char* s = buf.data();
+    auto skip_ws = [&]() { while (*s && (*s == ' ' || *s == '\t' || *s == '\n' || *s == '\r' || *s == '\v' || *s == '\f')) s++; };
+    auto token = [&]() -> bool {  // %s
+        skip_ws();
+        if (!*s) return false;
+        while (*s && !(*s == ' ' || *s == '\t' || *s == '\n' || *s == '\r' || *s == '\v' || *s == '\f')) s++;
+        return true;
+    };
+    auto read_int = [&](int* out) -> bool {  // %d
+        skip_ws();
+        char* e = nullptr;
+        long v = strtol(s, &e, 10);
+        if (e == s) return false;
+        s = e;
+        *out = (int)v;
+        return true;
+    };
+    auto read_dbl = [&](double* out) -> bool {  // %lf
+        skip_ws();
+        char* e = nullptr;
+        double v = strtod(s, &e);
+        if (e == s) return false;
+        s = e;
+        *out = v;
+        return true;
+    };
+    int ncols = 0, nrows = 0;
+    double xll = 0, yll = 0, cs = 0, nod = 0;
+    bool ok = token() && read_int(&ncols) && token() && read_int(&nrows) && token() &&
+              read_dbl(&xll) && token() && read_dbl(&yll) && token() && read_dbl(&cs) &&
+              token() && read_dbl(&nod);
+    if (!ok || ncols <= 0 || nrows <= 0) {
+        af::set_error("%s: malformed DEM header", path);
+        return AF_E_IO;
+    }
+    c->ncols = ncols;
+    c->nrows = nrows;
+    c->xll = xll;
+    c->yll = yll;
+    c->cellsize = cs;
+    c->nodata = nod;
+    c->dem.assign((size_t)ncols * nrows, 0.0);
+    size_t k = 0, n = c->dem.size();
+    for (; k < n; k++) {
+        double v;
+        if (!read_dbl(&v)) break;  // fscanf fails from here on
+        c->dem[k] = v;
+    }
+    c->have_dem = true;
+    return AF_OK;
+}
+
+// read_bci: 255-character line chunks (fgets into 256 bytes), point lines
+// `P x y TYPE [extra]` in world coordinates, side lines `S a b FREE`.
+int af_case_read_bci(af_case* c, const char* path) {
+    if (!c || !path) {
+        af::set_error("read_bci: null argument");
+        return AF_E_INVALID;
+    }
+    if (!c->have_dem) {
+        af::set_error("read_bci: read the DEM first");
+        return AF_E_STATE;
+    }
+    FILE* fp = fopen(path, "r");
+    if (!fp) {
+        af::set_error("Cannot open BCI file %s", path);
+        return AF_E_IO;
+    }
+    const double tly = c->yll + c->nrows * c->cellsize;  // io.cpp:170
+    char line[256];
+    while (fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#' || line[0] == '\n' || line[0] == '\r') continue;
+        char type = line[0];
+        if (type == 'P') {
+            double x = 0, y = 0;
+            char bctype[16] = {0}, extra[64] = {0};
+            int n = sscanf(line, "%c %lf %lf %15s %63s", &type, &x, &y, bctype, extra);
+            if (n < 4) continue;
+            Point pb;
+            pb.x = (int)((x - c->xll) / c->cellsize);  // io.cpp:185-186, truncation
+            pb.y = (int)((tly - y) / c->cellsize);
+            pb.value = 0.0;
+            if (strcmp(bctype, "QVAR") == 0 && n >= 5) {
+                pb.type = AF_BC_QVAR;
+                pb.name = extra;
+            } else if (strcmp(bctype, "QFIX") == 0 && n >= 5) {
+                pb.type = AF_BC_QFIX;
+                pb.value = atof(extra);
+            } else if (strcmp(bctype, "FREE") == 0) {
+                pb.type = AF_BC_FREE;
+            } else if (strcmp(bctype, "HFIX") == 0 && n >= 5) {
+                pb.type = AF_BC_HFIX;
+                pb.value = atof(extra);
+            } else if (strcmp(bctype, "HVAR") == 0 && n >= 5) {
+                pb.type = AF_BC_HVAR;
+                pb.name = extra;
+            } else {
+                continue;  // unknown type: skipped (io.cpp:214-216)
+            }
+            c->points.push_back(pb);
+        } else if (type == 'N' || type == 'S' || type == 'E' || type == 'W') {
+            int a = 0, b = 0;
+            char bctype[16] = {0};
+            int n = sscanf(line, "%c %d %d %15s", &type, &a, &b, bctype);
+            bool is_free = false;
+            if (n >= 4 && strcmp(bctype, "FREE") == 0) {
+                is_free = true;
+            } else if (n >= 2) {
+                char bctype2[16] = {0};
+                if (sscanf(line, "%c %15s", &type, bctype2) == 2 && strcmp(bctype2, "FREE") == 0)
+                    is_free = true;
+            }
+            if (is_free) c->line_sides.push_back(type);
+        }
+    }
+    fclose(fp);
+    return AF_OK;
+}
+
+// read_bdy: first line skipped; a line that starts with a QVAR/HVAR point's
+// name followed by end-of-line, space, CR or LF starts a section whose next
+// line holds the count and the count lines after it `value time`.  Only the
+// first point (in file order) with that name receives the section.
+int af_case_read_bdy(af_case* c, const char* path) {
+    if (!c || !path) {
+        af::set_error("read_bdy: null argument");
+        return AF_E_INVALID;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        af::set_error("Cannot open BDY file %s", path);
+        return AF_E_IO;
+    }
+    std::string line;
+    std::getline(f, line);
+    double val = 0.0, time = 0.0;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        for (Point& pb : c->points) {
+            if (pb.type != AF_BC_QVAR && pb.type != AF_BC_HVAR) continue;
+            const size_t L = pb.name.size();
+            if (line.compare(0, L, pb.name) == 0 && line.size() >= L &&
+                (line.size() == L || line[L] == ' ' || line[L] == '\n' || line[L] == '\r')) {
+                if (!std::getline(f, line)) line.clear();
+                int n = 0;
+                if (sscanf(line.c_str(), "%d", &n) != 1) n = 0;
+                for (int i = 0; i < n; i++) {
+                    if (!std::getline(f, line)) line.clear();
+                    double v2, t2;
+                    int got = sscanf(line.c_str(), "%lf %lf", &v2, &t2);
+                    if (got >= 1) val = v2;
+                    if (got >= 2) time = t2;
+                    pb.times.push_back(time);
+                    pb.values.push_back(val);
+                }
+                break;
+            }
+        }
+    }
+    return AF_OK;
+}
+
+// read_rain: first line skipped; rows `rate time`, rate in m/s.
+int af_case_read_rain(af_case* c, const char* path) {
+    if (!c || !path) {
+        af::set_error("read_rain: null argument");
+        return AF_E_INVALID;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        c->rain_enabled = false;
+        af::set_error("Cannot open rain file %s, rainfall disabled", path);
+        return AF_E_IO;
+    }
+    std::string line;
+    std::getline(f, line);
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        double rate, time;
+        if (sscanf(line.c_str(), "%lf %lf", &rate, &time) == 2) {
+            c->rain_t.push_back(time);
+            c->rain_r.push_back(rate);
+        }
+    }
+    if (!c->rain_t.empty()) c->rain_enabled = true;
+    return AF_OK;
+}
+
+// The input sequence of main.cpp:24-92.
+int af_case_load(af_case* c, const char* par_path) {
+    if (!c || !par_path) {
+        af::set_error("case_load: null argument");
+        return AF_E_INVALID;
+    }
+    int rc = af_read_par(par_path, &c->par);
+    if (rc) return rc;
+    if ((rc = af_case_read_dem(c, c->par.dem_file))) return rc;
+    if ((rc = af_case_read_bci(c, c->par.bci_file))) return rc;
+    if ((rc = af_case_read_bdy(c, c->par.bdy_file))) return rc;
+    if (c->par.rain_file[0]) (void)af_case_read_rain(c, c->par.rain_file);  // optional
+    return AF_OK;
+}
+
+int af_case_par(af_case* c, af_par* p) {
+    if (!c || !p) return AF_E_INVALID;
+    *p = c->par;
+    return AF_OK;
+}
+
+int af_case_dem_header(af_case* c, int32_t* ncols, int32_t* nrows, double* xll, double* yll,
+                       double* cellsize, double* nodata) {
+    if (!c || !c->have_dem) {
+        af::set_error("no DEM loaded");
+        return AF_E_STATE;
+    }
+    if (ncols) *ncols = c->ncols;
+    if (nrows) *nrows = c->nrows;
+    if (xll) *xll = c->xll;
+    if (yll) *yll = c->yll;
+    if (cellsize) *cellsize = c->cellsize;
+    if (nodata) *nodata = c->nodata;
+    return AF_OK;
+}
+
+int af_case_line_bcs(af_case* c, char* sides, int32_t cap, int32_t* n) {
+    if (!c || !n) return AF_E_INVALID;
+    *n = (int32_t)c->line_sides.size();
+    for (int32_t i = 0; i < *n && i < cap && sides; i++) sides[i] = c->line_sides[i];
+    return AF_OK;
+}
+
+int af_case_point(af_case* c, int32_t i, int32_t* x, int32_t* y, int32_t* type, double* value,
+                  int32_t* series_len, char* name, int32_t cap) {
+    if (!c || i < 0 || i >= (int32_t)c->points.size()) {
+        af::set_error("point %d out of range", i);
+        return AF_E_INVALID;
+    }
+    const Point& p = c->points[i];
+    if (x) *x = p.x;
+    if (y) *y = p.y;
+    if (type) *type = p.type;
+    if (value) *value = p.value;
+    if (series_len) *series_len = (int32_t)p.times.size();
+    if (name && cap > 0) copy_str(name, (size_t)cap, p.name);
+    return AF_OK;
+}
+
+int af_case_point_series(af_case* c, int32_t i, double* times, double* values, int32_t cap) {
+    if (!c || i < 0 || i >= (int32_t)c->points.size()) return AF_E_INVALID;
+    const Point& p = c->points[i];
+    for (size_t k = 0; k < p.times.size() && (int32_t)k < cap; k++) {
+        if (times) times[k] = p.times[k];
+        if (values) values[k] = p.values[k];
+    }
+    return AF_OK;
+}
+
+int af_case_rain(af_case* c, int32_t* enabled, double* times, double* rates, int32_t cap,
+                 int32_t* n) {
+    if (!c) return AF_E_INVALID;
+    if (enabled) *enabled = c->rain_enabled ? 1 : 0;
+    if (n) *n = (int32_t)c->rain_t.size();
+    for (size_t k = 0; k < c->rain_t.size() && (int32_t)k < cap; k++) {
+        if (times) times[k] = c->rain_t[k];
+        if (rates) rates[k] = c->rain_r[k];
+    }
+    return AF_OK;
+}
+
+// Scenario for the step: constants of main.cpp:115-127, line flags of
+// main.cpp:62-70, points in file order (main.cpp:77-85 keeps file order
+// within the flow and the other group, which the per-cell ordering of the
+// fused step preserves).
+int af_case_scenario(af_case* c, af_scenario_desc* d) {
+    if (!c || !d) return AF_E_INVALID;
+    if (!c->have_dem) {
+        af::set_error("no DEM loaded");
+        return AF_E_STATE;
+    }
+    memset(d, 0, sizeof(*d));
+    d->nx = c->ncols;
+    d->ny = c->nrows;
+    d->dx = c->cellsize;
+    d->nodata = c->nodata;
+    d->dem = c->dem.data();
+    d->fpn = c->par.fpn;
+    d->depth_thresh = c->par.depth_thresh;
+    d->max_hflow = 10.0;
+    d->init_tstep = c->par.init_tstep;
+    d->cfl = c->par.cfl;
+    d->g = 9.81;
+    d->theta = 1.0;
+    d->sim_time = c->par.sim_time;
+    d->mass_int = c->par.mass_int;
+    for (char s : c->line_sides) {
+        if (s == 'W') d->west_free = 1;
+        else if (s == 'E') d->east_free = 1;
+        else if (s == 'N') d->north_free = 1;
+        else if (s == 'S') d->south_free = 1;
+    }
+    c->px.clear(); c->py.clear(); c->ptype.clear(); c->poff.clear(); c->plen.clear();
+    c->pvalue.clear(); c->st.clear(); c->sv.clear();
+    for (const Point& p : c->points) {
+        c->px.push_back(p.x);
+        c->py.push_back(p.y);
+        c->ptype.push_back(p.type);
+        c->pvalue.push_back(p.value);
+        c->poff.push_back((int32_t)c->st.size());
+        c->plen.push_back((int32_t)p.times.size());
+        c->st.insert(c->st.end(), p.times.begin(), p.times.end());
+        c->sv.insert(c->sv.end(), p.values.begin(), p.values.end());
+    }
+    d->n_points = (int32_t)c->points.size();
+    d->pt_x = c->px.data();
+    d->pt_y = c->py.data();
+    d->pt_type = c->ptype.data();
+    d->pt_value = c->pvalue.data();
+    d->pt_series_off = c->poff.data();
+    d->pt_series_len = c->plen.data();
+    d->n_series = (int32_t)c->st.size();
+    d->series_t = c->st.data();
+    d->series_v = c->sv.data();
+    d->rain_enabled = c->rain_enabled ? 1 : 0;
+    d->n_rain = (int32_t)c->rain_t.size();
+    d->rain_t = c->rain_t.data();
+    d->rain_r = c->rain_r.data();
+    d->h0 = nullptr;
+    return AF_OK;
+}
+
+double af_interp_series(const double* times, const double* values, int32_t n, double empty_value,
+                        double t) {
+    if (n <= 0) return empty_value;
+    if (t <= times[0]) return values[0];
+    if (t >= times[n - 1]) return values[n - 1];
+    for (int32_t i = 0; i < n - 1; i++) {
+        if (t >= times[i] && t < times[i + 1]) {
+            double ratio = (t - times[i]) / (times[i + 1] - times[i]);
+            return values[i] + ratio * (values[i + 1] - values[i]);
+        }
+    }
+    return values[n - 1];
+}
+
+// write_asc: fixed header layout, NODATA_value always -9999, values "%.4f ".
+int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
+                 double yll, double cellsize) {
+    if (!path || !data || ncols <= 0 || nrows <= 0) return AF_E_INVALID;
+    FILE* fp = fopen(path, "w");
+    if (!fp) {
+        af::set_error("Cannot open %s for writing", path);
+        return AF_E_IO;
+    }
+    fprintf(fp, "ncols         %d\n", ncols);
+    fprintf(fp, "nrows         %d\n", nrows);
+    fprintf(fp, "xllcorner     %.1f\n", xll);
+    fprintf(fp, "yllcorner     %.1f\n", yll);
+    fprintf(fp, "cellsize      %.1f\n", cellsize);
+    fprintf(fp, "NODATA_value  -9999\n");
+    std::vector<char> row;
+    for (int32_t j = 0; j < nrows; j++) {
+        for (int32_t i = 0; i < ncols; i++) fprintf(fp, "%.4f ", data[i + (long long)j * ncols]);
+        fprintf(fp, "\n");
+    }
+    fclose(fp);
+    return AF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,295 @@
+/*
+ * adaptflood.h -- C ABI of the MI355X-native local-inertial flood step.
+ *
+ * Two layers share one set of device arithmetic:
+ *
+ *  1. The reference call surface (FP64, reference memory layout).  Each
+ *     af_<Name>_GPU replaces the reference's <Name>_GPU C++ function of the
+ *     same name (src/kernels.hpp:8-37; the point-source injection has no name
+ *     in the reference and lives inline at src/main.cpp:244-262).  A Kokkos
+ *     `RealView&` becomes a `double*` device pointer, an `IntView&` becomes an
+ *     `int32_t*` device pointer, and every call takes the HIP stream it is
+ *     enqueued on (NULL = the default stream).  Layout as in the reference:
+ *     H and DEM hold xsz*ysz cells, idx = i + j*xsz, row 0 is the north row;
+ *     Qx, Qy, Qxold, Qyold hold (xsz+1)*(ysz+1) faces with pitch xsz+1.
+ *     Like the reference, each call returns only after its work is done
+ *     (it synchronises the stream).
+ *
+ *  2. The fused production path (FP32 or FP64 state): a domain owns its state
+ *     in a caller-provided device arena and advances it by one fused HIP
+ *     kernel per step (fluxes, line and point boundaries, rain, depth update,
+ *     overrides and the next step's CFL maximum in one pass), optionally
+ *     replayed from a captured HIP graph.  Batched ensembles of ragged
+ *     domains advance together in one launch per step.
+ *
+ * Every entry point returns an int status: AF_OK (0) or a negative AF_E_*
+ * code; af_last_error() gives a message for the calling thread.  The library
+ * never allocates device memory: the caller owns every device buffer.
+ */
+#ifndef ADAPTFLOOD_AMD_H
+#define ADAPTFLOOD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* af_stream_t; /* a hipStream_t */
+
+enum {
+    AF_OK = 0,
+    AF_E_INVALID = -1,  /* bad argument (size, pointer, precision, index) */
+    AF_E_HIP = -2,      /* a HIP runtime call failed */
+    AF_E_IO = -3,       /* a file could not be opened or parsed */
+    AF_E_CAPACITY = -4, /* arena or table too small */
+    AF_E_STATE = -5     /* call out of order (e.g. run before load) */
+};
+
+const char* af_last_error(void);
+const char* af_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* 1. Reference call surface (FP64, reference layout)                      */
+/* ---------------------------------------------------------------------- */
+
+/* Bytes of device workspace af_FloodplainQ_GPU needs for its H-max
+ * reduction (the reference lets Kokkos allocate this implicitly). */
+#define AF_FLOODPLAINQ_WORKSPACE_BYTES 256
+
+/* Replaces UpdateQs_GPU (src/kernels.hpp:8-9, src/kernels.cpp:6-19):
+ * Qxold = Qx * (1/dx), Qyold = Qy * (1/dx) over all (xsz+1)*(ysz+1) faces. */
+int af_UpdateQs_GPU(const double* Qx, const double* Qy, double* Qxold, double* Qyold, int xsz,
+                    int ysz, double dx, af_stream_t stream);
+
+/* Replaces FloodplainQ_GPU (src/kernels.hpp:12-16, src/kernels.cpp:22-146):
+ * H-max over cells whose DEM != nodata, dt = Hmax > DepthThresh ?
+ * cfl*dx/sqrt(g*Hmax) : InitTstep written to *Tstep (host), then the interior
+ * x- and y-face inertial fluxes.  dy, dhlin, Qlimfact and dA are accepted
+ * and unused, as in the reference.  `workspace` is device memory of at least
+ * AF_FLOODPLAINQ_WORKSPACE_BYTES. */
+int af_FloodplainQ_GPU(const double* H, const double* DEM, double* Qx, double* Qy,
+                       const double* Qxold, const double* Qyold, int xsz, int ysz, double dx,
+                       double dy, double FPn, double DepthThresh, double MaxHflow, double dhlin,
+                       double* Tstep, double Qlimfact, double dA, double cfl, double g,
+                       double InitTstep, double theta, double nodata, void* workspace,
+                       af_stream_t stream);
+
+/* Replaces BCs_GPU (src/kernels.hpp:19-21, src/kernels.cpp:149-256): FREE or
+ * CLOSED edge faces from the pre-update H.  dt is unused, as in the reference. */
+int af_BCs_GPU(double* Qx, double* Qy, const double* H, const double* DEM, int xsz, int ysz,
+               double dx, double g, double dt, double nodata, int west_free, int east_free,
+               int north_free, int south_free, af_stream_t stream);
+
+/* Replaces ApplyPointBCs_GPU (src/kernels.hpp:24-25, src/kernels.cpp:259-277):
+ * bc_type 1 (FREE) sets H = 0, 2/3 (HFIX/HVAR) set H = bc_value.  Points that
+ * share a cell resolve in array order (the last one wins).  ncols, dx, dA and
+ * Tstep are unused, as in the reference. */
+int af_ApplyPointBCs_GPU(double* H, const int32_t* bc_idx, const int32_t* bc_type,
+                         const double* bc_value, int num_bcs, int ncols, double dx, double dA,
+                         double Tstep, af_stream_t stream);
+
+/* Replaces UpdateH_GPU (src/kernels.hpp:28, src/kernels.cpp:280-290):
+ * H += Tstep*(Qx_in - Qx_out + Qy_in - Qy_out)/dA, clamped at 0, all cells. */
+int af_UpdateH_GPU(double* H, const double* Qx, const double* Qy, int xsz, int ysz, double dA,
+                   double Tstep, af_stream_t stream);
+
+/* Replaces Rainfall_GPU (src/kernels.hpp:36-37, src/kernels.cpp:354-367):
+ * if rain_rate > 0, H += rain_rate*Tstep on cells whose DEM != nodata. */
+int af_Rainfall_GPU(double* H, const double* DEM, int xsz, int ysz, double rain_rate,
+                    double Tstep, double nodata, af_stream_t stream);
+
+/* Replaces the inline AddFlowBCs kernel (src/main.cpp:256-260):
+ * H[ps_idx[s]] += ps_flow[s], clamped at 0.  Sources that share a cell are
+ * applied in array order. */
+int af_AddFlowBCs_GPU(double* H, const int32_t* ps_idx, const double* ps_flow, int num_flow_bcs,
+                      af_stream_t stream);
+
+/* Replaces ComputeVelocity_GPU (src/kernels.hpp:31-33, src/kernels.cpp:293-351). */
+int af_ComputeVelocity_GPU(double* Vx_out, double* Vy_out, const double* H, const double* DEM,
+                           const double* Qx, const double* Qy, int xsz, int ysz, double dx,
+                           double nodata, af_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Fused path                                                           */
+/* ---------------------------------------------------------------------- */
+
+enum { AF_FP32 = 0, AF_FP64 = 1 };
+
+/* Point-boundary type codes of src/types.hpp:41-48 */
+enum { AF_BC_CLOSED = 0, AF_BC_FREE = 1, AF_BC_HFIX = 2, AF_BC_HVAR = 3, AF_BC_QFIX = 4,
+       AF_BC_QVAR = 5 };
+
+/* Everything one scenario needs, as host arrays.  Mirrors what main.cpp
+ * builds before its time loop (src/main.cpp:50-127, 166-194). */
+typedef struct af_scenario_desc {
+    int32_t nx, ny;          /* ncols, nrows */
+    double dx;               /* cellsize (dx = dy, dA = dx*dx; main.cpp:115) */
+    double nodata;           /* NODATA_value of the DEM */
+    const double* dem;       /* nx*ny, idx = i + j*nx, row 0 north */
+    double fpn;              /* Manning n (par.fpn) */
+    double depth_thresh;     /* par.depth_thresh */
+    double max_hflow;        /* 10.0 (main.cpp:118) */
+    double init_tstep;       /* par.init_tstep */
+    double cfl;              /* par.cfl */
+    double g;                /* 9.81 (main.cpp:123) */
+    double theta;            /* 1.0 (main.cpp:124); the fused path requires 1.0 */
+    double sim_time;         /* par.sim_time */
+    double mass_int;         /* par.mass_int (device-side mass reports) */
+    int32_t west_free, east_free, north_free, south_free; /* main.cpp:62-70 */
+    /* point boundaries in file order, any type (AF_BC_*); x/y are grid
+     * coordinates as read_bci computes them (src/io.cpp:185-187). */
+    int32_t n_points;
+    const int32_t* pt_x;
+    const int32_t* pt_y;
+    const int32_t* pt_type;
+    const double* pt_value;  /* QFIX/HFIX value; value used when a series is empty */
+    const int32_t* pt_series_off; /* offset into series_t/series_v (QVAR/HVAR) */
+    const int32_t* pt_series_len; /* 0 = empty series */
+    int32_t n_series;        /* total series entries */
+    const double* series_t;
+    const double* series_v;
+    /* uniform rain (m/s); rain_enabled follows read_rain (src/io.cpp:123-124) */
+    int32_t rain_enabled;
+    int32_t n_rain;
+    const double* rain_t;
+    const double* rain_r;
+    /* initial state (NULL = zeros, as main.cpp:156-160) in the reference layout */
+    const double* h0;        /* nx*ny */
+} af_scenario_desc;
+
+typedef struct af_run_info {
+    double t;                /* simulated time after the last executed step */
+    int64_t steps_done;      /* steps executed since af_domain_create */
+    double last_dt;          /* Tstep of the last executed step */
+    int32_t n_reports;       /* mass reports recorded so far */
+    int32_t finished;        /* t >= sim_time */
+    double hmax;             /* max H over valid cells of the current state */
+} af_run_info;
+
+typedef struct af_domain af_domain; /* opaque, host side */
+
+/* Tile shape: each wavefront owns a strip of 64*V columns (V = 4 at FP32,
+ * 2 at FP64) and marches down `tile_rows` rows.  0 = library default. */
+typedef struct af_tuning {
+    int32_t tile_rows;
+    int32_t waves_per_block; /* 1, 2 or 4 */
+    int32_t graph_steps;     /* steps per captured graph (multiple of 6); 0 = default */
+    int32_t dt_log_cap;      /* device dt log entries kept (0 = none) */
+    int32_t report_cap;      /* device mass-report records kept */
+} af_tuning;
+
+/* Device bytes the domain needs for `desc` at `precision`. */
+int af_domain_arena_bytes(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                          size_t* bytes);
+
+/* Builds a domain in `arena` (device memory of at least the queried size,
+ * 256-byte aligned) and uploads the scenario.  The arena stays owned by the
+ * caller and must outlive the domain. */
+int af_domain_create(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                     void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out);
+int af_domain_destroy(af_domain* d);
+
+/* Enqueues up to `max_steps` fused steps (each one a no-op once t >= sim_time).
+ * use_graph != 0 replays a captured graph of graph_steps steps. */
+int af_domain_advance(af_domain* d, int64_t max_steps, int use_graph, af_stream_t stream);
+/* Runs until t >= sim_time; checks completion every `check_every` steps. */
+int af_domain_run(af_domain* d, int64_t check_every, int use_graph, af_stream_t stream);
+/* Synchronises `stream` and reads the run counters. */
+int af_domain_info(af_domain* d, af_run_info* info, af_stream_t stream);
+
+/* Copies the current state out in the reference layout, converted to double:
+ * which = 0: H (nx*ny); 1: Qx; 2: Qy ((nx+1)*(ny+1), pitch nx+1, the
+ * padding entries zero); 3: Qxold; 4: Qyold.  `out` is host memory. */
+int af_domain_read(af_domain* d, int which, double* out, af_stream_t stream);
+/* Copies up to `cap` dt values of executed steps (from the device log). */
+int af_domain_read_dt_log(af_domain* d, double* out, int64_t cap, int64_t* n, af_stream_t stream);
+/* Mass reports as (t, Tstep, Vol, Area) quadruples, main.cpp:290-304. */
+int af_domain_read_reports(af_domain* d, double* out, int32_t cap, int32_t* n,
+                           af_stream_t stream);
+/* Device volume sum, FP64 with compensation: sum over all cells of H*dA. */
+int af_domain_volume(af_domain* d, double* vol, double* area, af_stream_t stream);
+/* Device pointer and pitch of the current H (for zero-copy consumers). */
+int af_domain_state_ptr(af_domain* d, int which, void** ptr, int64_t* pitch_elems);
+
+/* Batches: n ragged scenarios (each with its own grid, physics, points, rain,
+ * dt and clock) in one arena, advanced together by one launch per step over a
+ * table of tiles.  The af_domain_* calls above act on scenario 0 of a batch;
+ * the af_batch_* calls take the scenario index. */
+int af_batch_arena_bytes(const af_scenario_desc* descs, int n, int precision,
+                         const af_tuning* tuning, size_t* bytes);
+int af_batch_create(const af_scenario_desc* descs, int n, int precision, const af_tuning* tuning,
+                    void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out);
+int af_batch_size(af_domain* d, int* n_scenarios, int* n_tiles);
+/* *all_done = 1 when every scenario has reached its sim_time. */
+int af_batch_finished(af_domain* d, int* all_done, af_stream_t stream);
+int af_batch_info(af_domain* d, int scenario, af_run_info* info, af_stream_t stream);
+int af_batch_read(af_domain* d, int scenario, int which, double* out, af_stream_t stream);
+int af_batch_read_dt_log(af_domain* d, int scenario, double* out, int64_t cap, int64_t* n,
+                         af_stream_t stream);
+int af_batch_read_reports(af_domain* d, int scenario, double* out, int32_t cap, int32_t* n,
+                          af_stream_t stream);
+int af_batch_volume(af_domain* d, int scenario, double* vol, double* area, af_stream_t stream);
+/* Enqueues exactly one fused step without a graph (for timing one launch). */
+int af_batch_launch_step(af_domain* d, af_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Input formats (host): .par, .dem.ascii, .bci, .bdy, rain                */
+/* ---------------------------------------------------------------------- */
+
+typedef struct af_par {
+    char dem_file[512], bci_file[512], bdy_file[512], rain_file[512], stage_file[512],
+        output_dir[512];
+    double sim_time, save_int, mass_int, stage_int, init_tstep, cfl, fpn, depth_thresh, nodata;
+    int32_t output_format, voutput, async_output;
+} af_par;
+
+typedef struct af_case af_case; /* opaque parsed case */
+
+/* Parameters with the defaults of src/types.hpp:18-38. */
+void af_par_defaults(af_par* p);
+/* Replaces read_par (src/io.cpp:9-47); keys as the code reads them. */
+int af_read_par(const char* path, af_par* p);
+
+af_case* af_case_new(void);
+void af_case_free(af_case* c);
+/* Replaces read_dem (src/io.cpp:132-161). */
+int af_case_read_dem(af_case* c, const char* path);
+/* Replaces read_bci (src/io.cpp:164-254); needs the DEM header first. */
+int af_case_read_bci(af_case* c, const char* path);
+/* Replaces read_bdy (src/io.cpp:257-296). */
+int af_case_read_bdy(af_case* c, const char* path);
+/* Replaces read_rain (src/io.cpp:103-129). */
+int af_case_read_rain(af_case* c, const char* path);
+/* Runs the input sequence of main.cpp:24-92 for a .par file: par, DEM, BCI,
+ * BDY (all required) and the optional rain file. */
+int af_case_load(af_case* c, const char* par_path);
+/* Fills `desc` with pointers into the case (valid while the case lives),
+ * with the constants of main.cpp:115-127. */
+int af_case_scenario(af_case* c, af_scenario_desc* desc);
+int af_case_par(af_case* c, af_par* p);
+/* DEM header: ncols, nrows, xll, yll, cellsize, nodata. */
+int af_case_dem_header(af_case* c, int32_t* ncols, int32_t* nrows, double* xll, double* yll,
+                       double* cellsize, double* nodata);
+/* Line boundaries found (count of FREE sides and their letters, in file order). */
+int af_case_line_bcs(af_case* c, char* sides, int32_t cap, int32_t* n);
+/* Point i: grid x, y, type, value, series length, name (cap bytes). */
+int af_case_point(af_case* c, int32_t i, int32_t* x, int32_t* y, int32_t* type, double* value,
+                  int32_t* series_len, char* name, int32_t cap);
+int af_case_point_series(af_case* c, int32_t i, double* times, double* values, int32_t cap);
+/* Rain series (rate m/s, time s). */
+int af_case_rain(af_case* c, int32_t* enabled, double* times, double* rates, int32_t cap,
+                 int32_t* n);
+/* Linear interpolation of src/io.cpp:299-311 (host). */
+double af_interp_series(const double* times, const double* values, int32_t n, double empty_value,
+                        double t);
+/* ESRI-ASCII writer in the format of write_asc (src/io.cpp:314-334). */
+int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
+                 double yll, double cellsize);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADAPTFLOOD_AMD_H */

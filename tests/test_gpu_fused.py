@@ -1,0 +1,133 @@
+"""Fused HIP step vs the FP64 oracle (GPU).
+
+Bars: FP64 fused path max|dH| <= 1e-9 m against the oracle; FP32 fused path
+max|dH| <= 1e-3 m and relative volume error <= 1e-5.  Step counts and dt
+sequences must match the oracle's exactly at FP64.
+"""
+import numpy as np
+import pytest
+
+from adaptflood_amd import generators
+from adaptflood_amd.domain import Batch, Domain, make_tuning
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-9
+FP32_TOL = 1e-3
+
+
+def _small(seed, nx, ny, **kw):
+    return generators.small_case(nx, ny, seed=seed, **kw)
+
+
+def _faces_close(a, b, tol):
+    return float(np.max(np.abs(a - b))) <= tol
+
+
+@pytest.mark.parametrize("seed,nx,ny,rows", [
+    (1, 37, 29, 8), (2, 64, 64, 16), (3, 130, 70, 32), (4, 257, 19, 5), (5, 300, 300, 128),
+])
+def test_fp64_matches_oracle_small(gpu, seed, nx, ny, rows):
+    scn = _small(seed, nx, ny, sim_time=200.0)
+    print(f"case seed={seed} {nx}x{ny} rows={rows}", flush=True)
+    ref = oracle.run(scn)
+    d = Domain(scn, precision="fp64", tuning=make_tuning(tile_rows=rows, dt_log_cap=1 << 16))
+    d.run(check_every=50, graph=False)
+    info = d.info()
+    assert info["steps"] == ref["steps"]
+    assert info["t"] == pytest.approx(ref["t"], abs=1e-9)
+    np.testing.assert_allclose(d.dt_log(), ref["dt"], rtol=0, atol=1e-12)
+    assert float(np.max(np.abs(d.read("H") - ref["H"]))) <= FP64_TOL
+    assert _faces_close(d.read("Qx"), ref["Qx"], 1e-8)
+    assert _faces_close(d.read("Qy"), ref["Qy"], 1e-8)
+    assert _faces_close(d.read("Qxold"), ref["Qxold"], 1e-9)
+
+
+@pytest.mark.parametrize("edges", [(False, False, False, False), (True, False, True, False),
+                                   (False, True, False, True), (True, True, True, True)])
+def test_fp64_edges(gpu, edges):
+    scn = _small(11, 45, 38, edges=edges, sim_time=150.0)
+    print(f"edges={edges}", flush=True)
+    ref = oracle.run(scn)
+    d = Domain(scn, precision="fp64", tuning=make_tuning(tile_rows=7))
+    d.run(check_every=40, graph=False)
+    assert d.info()["steps"] == ref["steps"]
+    assert float(np.max(np.abs(d.read("H") - ref["H"]))) <= FP64_TOL
+    assert _faces_close(d.read("Qx"), ref["Qx"], 1e-8)
+    assert _faces_close(d.read("Qy"), ref["Qy"], 1e-8)
+
+
+def test_plane_case_fp64_full_run(gpu):
+    """256 x 256, 3600 s: the FP64 fused path against the oracle."""
+    scn = generators.plane_case()
+    ref = oracle.run(scn)
+    d = Domain(scn, precision="fp64", tuning=make_tuning(dt_log_cap=1 << 16))
+    d.run(check_every=360, graph=True)
+    info = d.info()
+    print(f"plane fp64 steps={info['steps']} oracle={ref['steps']}", flush=True)
+    assert info["steps"] == ref["steps"]
+    err = float(np.max(np.abs(d.read("H") - ref["H"])))
+    print(f"plane fp64 max|dH|={err:.3e}", flush=True)
+    assert err <= FP64_TOL
+    rep = d.reports()
+    assert rep.shape == ref["reports"].shape
+    np.testing.assert_allclose(rep[:, 0], ref["reports"][:, 0], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(rep[:, 1], ref["reports"][:, 1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rep[:, 2], ref["reports"][:, 2], rtol=1e-12)
+    np.testing.assert_allclose(rep[:, 3], ref["reports"][:, 3], rtol=0, atol=1e-9)
+
+
+def test_plane_case_fp32_full_run(gpu):
+    scn = generators.plane_case()
+    ref = oracle.run(scn)
+    d = Domain(scn, precision="fp32")
+    d.run(check_every=360, graph=True)
+    H = d.read("H")
+    err = float(np.max(np.abs(H - ref["H"])))
+    vol, _ = d.volume()
+    vref = float(np.sum(ref["H"]) * scn.dx * scn.dx)
+    rel = abs(vol - vref) / vref
+    print(f"plane fp32 max|dH|={err:.3e} rel vol err={rel:.3e} steps={d.info()['steps']} "
+          f"(oracle {ref['steps']})", flush=True)
+    assert err <= FP32_TOL
+    assert rel <= 1e-5
+
+
+def test_graph_equals_eager(gpu):
+    scn = _small(21, 90, 77, sim_time=300.0)
+    a = Domain(scn, precision="fp32", tuning=make_tuning(tile_rows=16, graph_steps=12))
+    a.run(check_every=30, graph=True)
+    b = Domain(scn, precision="fp32", tuning=make_tuning(tile_rows=16))
+    b.run(check_every=30, graph=False)
+    assert a.info()["steps"] == b.info()["steps"]
+    assert np.array_equal(a.read("H"), b.read("H"))
+    assert np.array_equal(a.read("Qx"), b.read("Qx"))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_tiling_invariance(gpu, prec):
+    """Results do not depend on the tile shape (bitwise)."""
+    scn = _small(31, 333, 141, sim_time=120.0)
+    outs = []
+    for rows, wpb in ((4, 1), (32, 2), (128, 4)):
+        d = Domain(scn, precision=prec, tuning=make_tuning(tile_rows=rows, waves_per_block=wpb))
+        d.run(check_every=25, graph=False)
+        outs.append((d.read("H"), d.read("Qx"), d.read("Qy")))
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert np.array_equal(x, y)
+
+
+def test_batch_equals_solo(gpu):
+    """Each scenario of a ragged batch gives bitwise the result it gives alone."""
+    scns = [_small(40 + k, nx, ny, sim_time=100.0 + 30 * k)
+            for k, (nx, ny) in enumerate([(50, 40), (130, 21), (17, 90), (260, 64)])]
+    b = Batch(scns, precision="fp32", tuning=make_tuning(tile_rows=16))
+    b.run(check_every=32, graph=True)
+    for k, s in enumerate(scns):
+        d = Domain(s, precision="fp32", tuning=make_tuning(tile_rows=16))
+        d.run(check_every=32, graph=True)
+        assert b.info(k)["steps"] == d.info()["steps"]
+        assert np.array_equal(b.read("H", k), d.read("H"))
+        assert np.array_equal(b.read("Qy", k), d.read("Qy"))

@@ -1,0 +1,335 @@
+"""Pins the FP64 oracle (oracle/oracle.c) before anything is compared with it.
+
+The reference publishes no vectors for this path and cannot be built here
+(it needs Kokkos), so each oracle function is checked against what the
+mathematics fixes: closed forms, hand-derived known answers, an independent
+long-double evaluation with geometric (2-D) indexing, and invariants that a
+dropped term, a wrong sign or index, or a transposed operand would break.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+G = 9.81
+NOD = -9999.0
+
+
+class S:  # minimal duck-typed scenario for oracle.run
+    def __init__(self, dem, dx, **kw):
+        self.dem = np.asarray(dem, dtype=np.float64)
+        self.dx = dx
+        self.nodata = kw.get("nodata", NOD)
+        self.fpn = kw.get("fpn", 0.035)
+        self.depth_thresh = kw.get("depth_thresh", 0.001)
+        self.max_hflow = kw.get("max_hflow", 10.0)
+        self.init_tstep = kw.get("init_tstep", 10.0)
+        self.cfl = kw.get("cfl", 0.7)
+        self.g = G
+        self.theta = kw.get("theta", 1.0)
+        self.sim_time = kw.get("sim_time", 100.0)
+        self.mass_int = kw.get("mass_int", 60.0)
+        self.west_free = kw.get("w", False)
+        self.east_free = kw.get("e", False)
+        self.north_free = kw.get("n", False)
+        self.south_free = kw.get("s", False)
+        self.points = kw.get("points", [])
+        self.rain_times = kw.get("rain_times")
+        self.rain_rates = kw.get("rain_rates")
+        self.h0 = kw.get("h0")
+
+
+class P:
+    def __init__(self, x, y, type, value=0.0, times=None, values=None):
+        self.x, self.y, self.type, self.value = x, y, type, value
+        self.times, self.values = times, values
+
+
+# ---------------------------------------------------------------- dt (a1)
+
+def test_dt_closed_form_and_nodata_exclusion():
+    H = np.array([0.0, 0.5, 9.0, 0.2])
+    dem = np.array([1.0, 2.0, NOD, 0.0])
+    hmax = oracle.find_hmax(H, dem, NOD)
+    assert hmax == 0.5  # the NODATA cell's 9 m is excluded
+    dt = oracle.lib().orc_dt_from_hmax(hmax, 0.001, 0.7, 10.0, G, 10.0)
+    assert dt == pytest.approx(0.7 * 10.0 / math.sqrt(G * 0.5), rel=1e-15)
+
+
+def test_dt_dry_and_all_nodata():
+    assert oracle.lib().orc_dt_from_hmax(0.0, 0.001, 0.7, 10.0, G, 10.0) == 10.0
+    assert oracle.lib().orc_dt_from_hmax(0.001, 0.001, 0.7, 10.0, G, 7.5) == 7.5  # not > thresh
+    H = np.array([3.0, 4.0])
+    hmax = oracle.find_hmax(H, np.array([NOD, NOD]), NOD)
+    assert hmax == -np.finfo(np.float64).max
+    assert oracle.lib().orc_dt_from_hmax(hmax, 0.001, 0.7, 10.0, G, 10.0) == 10.0
+
+
+# ---------------------------------------------------------------- one face (a2/a3)
+
+def _flux_ld(z0, z1, h0, h1, q0, qup, qdown, qt, dx, fn, thresh, maxh, dt, theta):
+    """Independent long-double evaluation of the local-inertial face flux
+    (de Almeida & Bates 2012, semi-implicit friction)."""
+    L = np.longdouble
+    z0, z1, h0, h1 = L(z0), L(z1), L(h0), L(h1)
+    eta0, eta1 = z0 + h0, z1 + h1
+    hflow = min(max(max(eta0, eta1) - max(z0, z1), L(0)), L(maxh))
+    if not hflow > thresh:
+        return 0.0
+    slope = (eta1 - eta0) / L(dx)          # water-surface slope toward the downstream cell
+    qv = np.sqrt(L(q0) ** 2 + L(qt) ** 2)
+    num = L(theta) * L(q0) + L(0.5) * (1 - L(theta)) * (L(qup) + L(qdown)) - L(G) * L(dt) * hflow * slope
+    den = 1 + L(G) * L(dt) * L(fn) ** 2 * qv / hflow ** (L(7) / L(3))
+    q = num / den * L(dx)
+    if q * (eta0 - eta1) < 0:
+        q = (L(q0) - L(G) * L(dt) * hflow * slope) / den * L(dx)
+    return float(q)
+
+
+@pytest.mark.parametrize("case", [
+    # z0, z1, h0, h1, q0, qup, qdown, qt, theta
+    (1.0, 0.9, 0.5, 0.3, 0.02, 0.0, 0.0, 0.01, 1.0),
+    (0.0, 0.0, 0.2, 0.6, -0.05, 0.0, 0.0, 0.03, 1.0),
+    (2.0, 2.5, 1.0, 0.2, 0.0, 0.0, 0.0, 0.0, 1.0),
+    (5.0, 1.0, 12.0, 0.0, 0.3, 0.0, 0.0, 0.1, 1.0),       # hflow clamped at MaxHflow
+    (1.0, 0.9, 0.5, 0.3, 0.02, 0.04, -0.01, 0.01, 0.7),   # theta != 1 uses qup/qdown
+    (0.0, 0.0, 1.0, 0.99, -0.8, -0.9, -0.9, 0.0, 0.5),    # sign-reversal branch
+])
+def test_face_flux_vs_long_double(case):
+    z0, z1, h0, h1, q0, qup, qdown, qt, theta = case
+    dx, fn, dt = 10.0, 0.035, 1.7
+    got = oracle.face_flux(z0, z1, h0, h1, q0, qup, qdown, qt, dx, fn, 0.001, 10.0, dt, G, theta)
+    want = _flux_ld(z0, z1, h0, h1, q0, qup, qdown, qt, dx, fn, 0.001, 10.0, dt, theta)
+    assert got == pytest.approx(want, rel=1e-13, abs=1e-300)
+    assert got != 0.0
+
+
+def test_face_flux_dry_and_threshold():
+    args = dict(dx=10.0, fn=0.035, DepthThresh=0.001, MaxHflow=10.0, dt=1.0, g=G, theta=1.0)
+    # both cells dry
+    assert oracle.face_flux(1.0, 1.0, 0.0, 0.0, 0.5, 0, 0, 0, *args.values()) == 0.0
+    # hflow exactly the threshold: not wet (strict >)
+    assert oracle.face_flux(0.0, 0.0, 0.001, 0.0, 0.5, 0, 0, 0, *args.values()) == 0.0
+
+
+def test_flux_indexing_against_geometric_2d():
+    """orc_floodplain_q's flat-index arithmetic against a 2-D formulation:
+    x face (j, f) lies between cells (j, f-1) and (j, f); its transverse
+    average takes the four y faces at columns f-1, f of face rows j, j+1."""
+    rng = np.random.default_rng(3)
+    nx, ny, dx = 7, 5, 4.0
+    dem = rng.uniform(0, 1, (ny, nx))
+    dem[2, 3] = NOD
+    H = rng.uniform(0, 1.5, (ny, nx))
+    Qxo = rng.normal(0, 0.2, (ny + 1, nx + 1))
+    Qyo = rng.normal(0, 0.2, (ny + 1, nx + 1))
+    Qx = np.zeros((ny + 1) * (nx + 1))
+    Qy = np.zeros_like(Qx)
+    dt = oracle.floodplain_q(H.ravel(), dem.ravel(), Qx, Qy, Qxo.ravel(), Qyo.ravel(), nx, ny, dx,
+                             0.035, 0.001, 10.0, 0.7, G, 10.0, 0.8, NOD)
+    assert dt == pytest.approx(0.7 * dx / math.sqrt(G * H[dem != NOD].max()), rel=1e-15)
+    Qx = Qx.reshape(ny + 1, nx + 1)
+    Qy = Qy.reshape(ny + 1, nx + 1)
+    for j in range(ny):
+        for f in range(1, nx):
+            a, b = (j, f - 1), (j, f)
+            if dem[a] == NOD or dem[b] == NOD:
+                assert Qx[j, f] == 0.0
+                continue
+            qt = (Qyo[j, f - 1] + Qyo[j, f] + Qyo[j + 1, f - 1] + Qyo[j + 1, f]) / 4
+            qup = Qxo[j, f - 1] if f > 1 else 0.0
+            qdn = Qxo[j, f + 1] if f < nx - 1 else 0.0
+            want = _flux_ld(dem[a], dem[b], H[a], H[b], Qxo[j, f], qup, qdn, qt, dx, 0.035,
+                            0.001, 10.0, dt, 0.8)
+            assert Qx[j, f] == pytest.approx(want, rel=1e-12, abs=1e-300)
+    for g_ in range(1, ny):
+        for i in range(nx):
+            a, b = (g_ - 1, i), (g_, i)
+            if dem[a] == NOD or dem[b] == NOD:
+                assert Qy[g_, i] == 0.0
+                continue
+            qt = (Qxo[g_ - 1, i] + Qxo[g_ - 1, i + 1] + Qxo[g_, i] + Qxo[g_, i + 1]) / 4
+            qup = Qyo[g_ - 1, i] if g_ > 1 else 0.0
+            qdn = Qyo[g_ + 1, i] if g_ < ny - 1 else 0.0
+            want = _flux_ld(dem[a], dem[b], H[a], H[b], Qyo[g_, i], qup, qdn, qt, dx, 0.035,
+                            0.001, 10.0, dt, 0.8)
+            assert Qy[g_, i] == pytest.approx(want, rel=1e-12, abs=1e-300)
+
+
+# ---------------------------------------------------------------- edges (a4)
+
+def test_edge_closed_forms():
+    nx, ny, dx = 4, 3, 2.0
+    dem = np.zeros((ny, nx))
+    H = np.full((ny, nx), 0.5)
+    H[:, 1] = 0.1  # west inner neighbour lower: outflow at W
+    H[0, 2] = 0.001  # exactly at the 0.001 threshold on the north edge
+    dem[2, 3] = NOD
+    nq = (nx + 1) * (ny + 1)
+    Qx = np.full(nq, 7.0)
+    Qy = np.full(nq, 7.0)
+    oracle.bcs(Qx, Qy, H.ravel(), dem.ravel(), nx, ny, dx, G, NOD, 1, 1, 1, 1)
+    Qx = Qx.reshape(ny + 1, nx + 1)
+    Qy = Qy.reshape(ny + 1, nx + 1)
+    F = lambda h: h * math.sqrt(G * h) * dx * 0.5  # noqa: E731
+    for j in range(ny):
+        assert Qx[j, 0] == pytest.approx(-F(0.5), rel=1e-15)       # W: -F when Sf < 0
+        assert Qx[j, nx] == (0.0 if j == 2 else pytest.approx(F(0.5), rel=1e-15))  # E, NODATA
+    assert Qy[0, 2] == 0.0                                          # h0 == 0.001: neither branch
+    assert Qy[0, 0] == pytest.approx(-F(0.5), rel=1e-15)            # N: -F
+    assert Qy[ny, 0] == pytest.approx(F(0.5), rel=1e-15)            # S: +F
+    assert Qy[ny, 3] == 0.0                                         # S over a NODATA cell
+    # FREE sides leave their padding entry alone; CLOSED sides zero it
+    assert Qx[ny, 0] == 7.0 and Qy[0, nx] == 7.0
+    Qx2 = np.full(nq, 7.0)
+    Qy2 = np.full(nq, 7.0)
+    oracle.bcs(Qx2, Qy2, H.ravel(), dem.ravel(), nx, ny, dx, G, NOD, 0, 0, 0, 0)
+    Qx2 = Qx2.reshape(ny + 1, nx + 1)
+    Qy2 = Qy2.reshape(ny + 1, nx + 1)
+    assert np.all(Qx2[:, 0] == 0) and np.all(Qx2[:, nx] == 0)
+    assert np.all(Qy2[0, :] == 0) and np.all(Qy2[ny, :] == 0)
+
+
+def test_west_edge_needs_falling_surface():
+    nx, ny, dx = 3, 2, 1.0
+    dem = np.zeros((ny, nx))
+    H = np.array([[0.5, 0.7, 0.0], [0.5, 0.5, 0.0]])  # row 0 rises inward, row 1 flat
+    Qx = np.zeros((ny + 1) * (nx + 1))
+    Qy = np.zeros_like(Qx)
+    oracle.bcs(Qx, Qy, H.ravel(), dem.ravel(), nx, ny, dx, G, NOD, 1, 0, 0, 0)
+    assert Qx[0] == 0.0 and Qx[nx + 1] == 0.0
+
+
+# ---------------------------------------------------------------- sources, rain, update (a5-a9)
+
+def test_source_volume_and_duplicate_order():
+    H = np.array([0.1, 0.0, 0.0])
+    oracle.add_flow_bcs(H, np.array([0, 0, 2], dtype=np.int32), np.array([-0.3, 0.5, 0.25]))
+    # sequential with a clamp after each: 0.1-0.3 -> 0, then +0.5 (a summed update would give 0.3)
+    assert H[0] == 0.5 and H[2] == 0.25
+    # one step on a closed flat basin: a source of v m^2/s adds v*dx*dt m^3
+    dx, v = 10.0, 0.5
+    s = S(np.zeros((5, 5)), dx, points=[P(2, 2, oracle.QFIX, v)], sim_time=1e-9)
+    r = oracle.run(s, max_steps=1)
+    assert r["steps"] == 1 and r["dt"][0] == 10.0
+    assert float(np.sum(r["H"])) * dx * dx == pytest.approx(v * dx * 10.0, rel=1e-15)
+
+
+def test_interp_kats():
+    t = np.array([0.0, 100.0, 100.0, 200.0])
+    v = np.array([1.0, 3.0, 10.0, 20.0])
+    assert oracle.interp(t, v, 9.0, -5.0) == 1.0          # before the first time
+    assert oracle.interp(t, v, 9.0, 0.0) == 1.0
+    assert oracle.interp(t, v, 9.0, 50.0) == 2.0          # linear
+    assert oracle.interp(t, v, 9.0, 100.0) == 10.0        # duplicate time: the later bracket
+    assert oracle.interp(t, v, 9.0, 150.0) == 15.0
+    assert oracle.interp(t, v, 9.0, 250.0) == 20.0        # clamped at the end
+    assert oracle.interp(np.zeros(0), np.zeros(0), 9.0, 1.0) == 9.0  # empty: the fixed value
+
+
+def test_rain_on_closed_flat_basin():
+    dx, r = 5.0, 2e-5
+    dem = np.zeros((6, 9))
+    dem[0, 0] = NOD
+    s = S(dem, dx, rain_times=np.array([0.0]), rain_rates=np.array([r]), sim_time=500.0)
+    out = oracle.run(s)
+    valid = dem != NOD
+    assert np.all(out["H"][~valid] == 0.0)
+    vol = float(np.sum(out["H"])) * dx * dx
+    assert vol == pytest.approx(r * out["t"] * valid.sum() * dx * dx, rel=1e-12)
+
+
+def test_update_h_divergence_and_clamp():
+    nx, ny = 2, 2
+    P_ = nx + 1
+    Qx = np.zeros((ny + 1) * P_)
+    Qy = np.zeros_like(Qx)
+    Qx[1 + 0 * P_] = 3.0     # face between (0,0) and (1,0): flow east
+    Qy[0 + 1 * P_] = -2.0    # face between (0,0) and (0,1): flow north
+    H = np.array([1.0, 1.0, 1.0, 0.001])
+    oracle.update_h(H, Qx, Qy, nx, ny, 4.0, 2.0)
+    assert H[0] == pytest.approx(1.0 + 2.0 * (0 - 3.0 + 0 - (-2.0)) / 4.0)
+    assert H[1] == pytest.approx(1.0 + 2.0 * 3.0 / 4.0)
+    assert H[2] == pytest.approx(1.0 + 2.0 * (-2.0) / 4.0)
+    assert H[3] == 0.001
+
+
+def test_overrides_last_wins():
+    H = np.array([1.0, 1.0, 1.0])
+    oracle.apply_point_bcs(H, np.array([0, 0, 2, 1], dtype=np.int32),
+                           np.array([2, 1, 3, 2], dtype=np.int32), np.array([0.4, 9.0, 0.7, 0.3]))
+    assert list(H) == [0.0, 0.3, 0.7]
+
+
+def test_update_qs_is_scaled_copy():
+    Qx = np.array([1.0, -3.0, 0.5, 7.0])
+    Qy = np.array([2.0, 0.0, -1.0, 4.0])
+    a, b = np.zeros(4), np.zeros(4)
+    oracle.update_qs(Qx, Qy, a, b, 1, 1, 3.0)
+    assert np.array_equal(a, Qx * (1.0 / 3.0)) and np.array_equal(b, Qy * (1.0 / 3.0))
+
+
+# ---------------------------------------------------------------- invariants (whole step)
+
+def test_lake_at_rest_is_exact():
+    rng = np.random.default_rng(5)
+    dem = rng.integers(0, 64, (12, 15)) / 64.0          # dyadic: eta - z and z + h are exact
+    eta = 2.0
+    s = S(dem, 10.0, h0=eta - dem, sim_time=600.0)
+    out = oracle.run(s)
+    assert out["steps"] > 10
+    assert np.array_equal(out["H"], eta - dem)
+    assert np.all(out["Qx"] == 0.0) and np.all(out["Qy"] == 0.0)
+
+
+def test_closed_domain_conserves_mass():
+    rng = np.random.default_rng(6)
+    dem = rng.uniform(0, 0.05, (20, 24))
+    h0 = 1.0 + rng.uniform(0, 0.05, dem.shape)
+    s = S(dem, 10.0, h0=h0, sim_time=300.0)
+    out = oracle.run(s)
+    assert np.min(out["H"]) > 0.1     # no clamp ever fired
+    v0, v1 = float(np.sum(h0)), float(np.sum(out["H"]))
+    assert abs(v1 - v0) / v0 < 1e-13
+
+
+def test_transpose_symmetry():
+    """Swapping the x and y axes of the inputs swaps Qx and Qy of the result
+    (E and S edges share one formula; W differs from N in the reference)."""
+    rng = np.random.default_rng(8)
+    n = 14
+    dem = rng.uniform(0, 1, (n, n))
+    h0 = rng.uniform(0.2, 0.8, (n, n))
+    a = oracle.run(S(dem, 5.0, h0=h0, sim_time=200.0, e=True, s=True))
+    b = oracle.run(S(dem.T.copy(), 5.0, h0=h0.T.copy(), sim_time=200.0, e=True, s=True))
+    assert a["steps"] == b["steps"]
+    np.testing.assert_allclose(a["H"], b["H"].T, rtol=0, atol=1e-10)  # summation order only
+    qxa = a["Qx"].reshape(n + 1, n + 1)[:n, :]      # x faces (j, f)
+    qyb = b["Qy"].reshape(n + 1, n + 1)[:, :n]      # y faces (g, i)
+    np.testing.assert_allclose(qxa, qyb.T, rtol=0, atol=1e-9)
+
+
+def test_manning_normal_depth_on_plane():
+    """Uniform inflow q per unit width down a plane of slope S settles at the
+    normal depth h = (q n / sqrt(S))^(3/5) (q_new = q_old in the face update)."""
+    nx, ny, dx, slope, n_, q = 240, 3, 10.0, 0.001, 0.035, 0.1
+    x = np.arange(nx) * dx
+    dem = np.repeat((slope * (x[-1] - x))[None, :], ny, 0)
+    pts = [P(0, j, oracle.QFIX, q) for j in range(ny)]
+    out = oracle.run(S(dem, dx, fpn=n_, points=pts, e=True, sim_time=40000.0))
+    h = out["H"][1, 40:100].mean()
+    h_normal = (q * n_ / math.sqrt(slope)) ** 0.6
+    assert h == pytest.approx(h_normal, rel=0.02)
+
+
+def test_step_count_and_time_sequence():
+    """t advances by each step's dt, the loop stops at the first t >= sim_time
+    (no clamp of the last step), and dt follows the H maximum."""
+    s = S(np.zeros((4, 4)), 10.0, points=[P(1, 1, oracle.QFIX, 2.0)], sim_time=95.0)
+    out = oracle.run(s)
+    assert np.isclose(out["t"], np.sum(out["dt"]), rtol=0, atol=1e-12)
+    assert out["t"] >= 95.0 and out["t"] - out["dt"][-1] < 95.0
+    assert out["dt"][0] == 10.0          # dry start
+    assert out["dt"][1] < 10.0           # the source wetted a cell above the threshold
