@@ -75,6 +75,31 @@ __device__ __forceinline__ double face_flux_ref_t1(double z0, double z1, double 
     return Q;
 }
 
+// face_flux_ref_t1 on stored discharges: Q0 is the face's Q and a..d the
+// four transverse Q of the average; both become q_old = Q * sc (sc = 1/dx)
+// exactly as UpdateQs_GPU scales them (kernels.cpp:8-16), only on wet faces.
+__device__ __forceinline__ double face_flux_ref_t1_q(double z0, double z1, double h0, double h1,
+                                                     double Q0, double a, double b, double c,
+                                                     double d, double sc,
+                                                     const FluxConstsF64& k) {
+    double y0 = z0 + h0, y1 = z1 + h1;
+    double dh = y0 - y1;
+    double hflow = getmax(y0, y1) - getmax(z0, z1);
+    hflow = getmax(hflow, 0.0);
+    hflow = getmin(hflow, k.maxh);
+    double Q = 0.0;
+    if (hflow > k.thresh) {
+        const double q0 = Q0 * sc;
+        const double qt = (((a * sc + b * sc) + c * sc) + d * sc) / 4.0;
+        double Sf = -dh / k.dx;
+        double qvect = sqrt(q0 * q0 + qt * qt);
+        double num = (k.theta * q0) - (k.gdt * hflow * Sf);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        Q = num / den * k.dx;
+    }
+    return Q;
+}
+
 // Constants of one step's face fluxes, FP32 form.
 struct FluxConstsF32 {
     float dx;
@@ -82,6 +107,8 @@ struct FluxConstsF32 {
     float maxh;
     float gdt_dx;   // g*dt/dx: num = q0 + (g*dt/dx)*hflow*dh
     float gdt_n2;   // g*dt*n^2: den = 1 + g*dt*n^2*|q|*hflow^(-7/3)
+    float sc;       // 1/dx: Q -> q_old
+    float sc4;      // 0.25/dx: sum of four Q -> transverse q_old average
 };
 
 // FP32 form of face_flux_ref_t1.  Same physics, rearranged for the VALU:
@@ -98,6 +125,28 @@ __device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, flo
     hflow = getmin(hflow, k.maxh);
     float Q = 0.0f;
     if (hflow > k.thresh) {
+        float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
+        float num = q0 + k.gdt_dx * hflow * dh;
+        float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
+        float den = 1.0f + k.gdt_n2 * qvect * p;
+        Q = num * k.dx * __builtin_amdgcn_rcpf(den);
+    }
+    return Q;
+}
+
+// face_flux_f32 on stored discharges (see face_flux_ref_t1_q).
+__device__ __forceinline__ float face_flux_f32_q(float z0, float z1, float h0, float h1, float Q0,
+                                                 float a, float b, float c, float d,
+                                                 const FluxConstsF32& k) {
+    float y0 = z0 + h0, y1 = z1 + h1;
+    float dh = y0 - y1;
+    float hflow = getmax(y0, y1) - getmax(z0, z1);
+    hflow = getmax(hflow, 0.0f);
+    hflow = getmin(hflow, k.maxh);
+    float Q = 0.0f;
+    if (hflow > k.thresh) {
+        const float q0 = Q0 * k.sc;
+        const float qt = (((a + b) + c) + d) * k.sc4;
         float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
         float num = q0 + k.gdt_dx * hflow * dh;
         float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
@@ -162,8 +211,8 @@ __host__ __device__ __forceinline__ float fkey_inv_f(unsigned int k) {
 
 // Linear interpolation of src/io.cpp:299-311: clamped at both ends, bracket
 // [t_i, t_{i+1}), empty series -> empty_value.
-__host__ __device__ __forceinline__ double interp_series(const double* times,
-                                                         const double* values, int n,
+template <typename PD>
+__host__ __device__ __forceinline__ double interp_series(PD times, PD values, int n,
                                                          double empty_value, double t) {
     if (n <= 0) return empty_value;
     if (t <= times[0]) return values[0];

@@ -58,7 +58,7 @@ struct HostScn {
 
 struct af_domain {
     int precision = AF_FP32;
-    int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36;
+    int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36, variant = 0;
     long long dt_log_cap = 0;
     int report_cap = 64;
     char* arena = nullptr;
@@ -228,10 +228,11 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
         if (tun->graph_steps > 0) D->graph_steps = tun->graph_steps;
         if (tun->dt_log_cap > 0) D->dt_log_cap = tun->dt_log_cap;
         if (tun->report_cap > 0) D->report_cap = tun->report_cap;
+        D->variant = tun->variant;
     }
     AF_REQUIRE(D->R >= 1 && D->R <= 65536, AF_E_INVALID, "tile_rows %d", D->R);
-    AF_REQUIRE(D->wpb == 1 || D->wpb == 2 || D->wpb == 4, AF_E_INVALID, "waves_per_block %d",
-               D->wpb);
+    AF_REQUIRE(D->wpb == 1 || D->wpb == 4, AF_E_INVALID, "waves_per_block %d (1 or 4)", D->wpb);
+    AF_REQUIRE(D->variant >= 0 && D->variant <= 2, AF_E_INVALID, "variant %d (0..2)", D->variant);
     AF_REQUIRE(D->graph_steps % 6 == 0 && D->graph_steps <= 6000, AF_E_INVALID,
                "graph_steps %d must be a positive multiple of 6", D->graph_steps);
     return AF_OK;
@@ -417,26 +418,35 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     return AF_OK;
 }
 
-template <typename T, int V, bool F64>
-int launch_step_t(af_domain* D, int kmod6, hipStream_t st) {
+template <typename T, int V, bool F64, int OCC, int PF>
+int launch_step_v(af_domain* D, int kmod6, hipStream_t st) {
     const int wpb = D->wpb;
     const unsigned blocks = (unsigned)((D->ntiles + wpb - 1) / wpb);
     if (wpb == 4)
-        hipLaunchKernelGGL((k_step<T, V, F64, 4>), dim3(blocks), dim3(256), 0, st, D->d_scns,
-                           D->d_tiles, D->ntiles, kmod6);
-    else if (wpb == 2)
-        hipLaunchKernelGGL((k_step<T, V, F64, 2>), dim3(blocks), dim3(128), 0, st, D->d_scns,
-                           D->d_tiles, D->ntiles, kmod6);
+        hipLaunchKernelGGL((k_step<T, V, F64, 4, OCC, PF>), dim3(blocks), dim3(256), 0, st,
+                           D->d_scns, D->d_tiles, D->ntiles, kmod6);
     else
-        hipLaunchKernelGGL((k_step<T, V, F64, 1>), dim3(blocks), dim3(64), 0, st, D->d_scns,
-                           D->d_tiles, D->ntiles, kmod6);
+        hipLaunchKernelGGL((k_step<T, V, F64, 1, OCC, PF>), dim3(blocks), dim3(64), 0, st,
+                           D->d_scns, D->d_tiles, D->ntiles, kmod6);
     AF_HIP(hipGetLastError());
     return AF_OK;
 }
 
+// Kernel variants: rows of loads each wavefront keeps in flight (PF) and the
+// occupancy (waves per SIMD) the register allocation targets.
 int launch_step(af_domain* D, int kmod6, hipStream_t st) {
-    if (D->precision == AF_FP64) return launch_step_t<double, 2, true>(D, kmod6, st);
-    return launch_step_t<float, 4, false>(D, kmod6, st);
+    if (D->precision == AF_FP64) {
+        switch (D->variant) {
+            case 1: return launch_step_v<double, 2, true, 2, 2>(D, kmod6, st);
+            case 2: return launch_step_v<double, 2, true, 3, 1>(D, kmod6, st);
+            default: return launch_step_v<double, 2, true, 2, 1>(D, kmod6, st);
+        }
+    }
+    switch (D->variant) {
+        case 1: return launch_step_v<float, 4, false, 3, 2>(D, kmod6, st);
+        case 2: return launch_step_v<float, 4, false, 4, 2>(D, kmod6, st);
+        default: return launch_step_v<float, 4, false, 4, 1>(D, kmod6, st);
+    }
 }
 
 int build_graph(af_domain* D, hipStream_t st) {

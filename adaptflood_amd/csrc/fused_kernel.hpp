@@ -92,10 +92,19 @@ struct TileDesc {
     int scn, strip, seg, first;
 };
 
+// V adjacent values of one row, moved as one 8- or 16-byte access
 template <typename T, int V>
-struct alignas(sizeof(T) * V) Pack {
-    T v[V];
-};
+using Vec = T __attribute__((ext_vector_type(V)));
+
+// Pointers loaded from the scenario table are generic; the kernel works on
+// them as global-memory pointers so that loads and stores are global_* (not
+// flat_*) instructions and uniform loads can go through the scalar unit.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glob(const void* p) {
+    return (gptr<T>)(const_cast<void*>(p));
+}
 
 template <typename T>
 __device__ __forceinline__ T lane_prev(T x, T halo, int lane) {
@@ -118,44 +127,287 @@ __device__ __forceinline__ void kahan_add(double& s, double& c, double x) {
 
 __device__ __forceinline__ unsigned long long key_neg_max() { return fkey(-DBL_MAX); }
 
-// One row of cells (and halos) of one array at row r.
+// One row of one array at row r (r >= 0), V columns from c0.  The load is
+// always issued, from a clamped in-bounds address: a load under a branch
+// would leave the wait counter unknown and make the next use wait for every
+// load in flight.  Values read for rows or columns outside the array are
+// never consumed: every face and cell that would use them is masked by its
+// own row or column test.
 template <typename T, int V>
-__device__ __forceinline__ void load_row(T (&dst)[V], const T* __restrict__ base, long long P,
+__device__ __forceinline__ void load_row(T (&dst)[V], gptr<const T> base, long long P,
                                          int r, int rows, int c0) {
-    if (r >= 0 && r < rows && c0 < P) {
-        Pack<T, V> p = *reinterpret_cast<const Pack<T, V>*>(base + (long long)r * P + c0);
+    const int rr = (r < rows) ? r : rows - 1;
+    const int cc = (c0 < P) ? c0 : 0;
+    Vec<T, V> p = *(gptr<const Vec<T, V>>)(base + (long long)rr * P + cc);
 #pragma unroll
-        for (int v = 0; v < V; v++) dst[v] = p.v[v];
-    } else {
-#pragma unroll
-        for (int v = 0; v < V; v++) dst[v] = T(0);
-    }
+    for (int v = 0; v < V; v++) dst[v] = p[v];
 }
+// One value at (r, col); col must be a valid column.
 template <typename T>
-__device__ __forceinline__ T load_one(const T* __restrict__ base, long long P, int r, int rows,
-                                      int col, bool ok) {
-    return (ok && r >= 0 && r < rows) ? base[(long long)r * P + col] : T(0);
+__device__ __forceinline__ T load_one(gptr<const T> base, long long P, int r, int rows, int col) {
+    const int rr = (r < rows) ? r : rows - 1;
+    return base[(long long)rr * P + col];
 }
 
-template <typename T, int V, bool F64>
+// One row of the sliding window: V columns of H, z, Qx and Qy per lane, plus
+// the strip-edge value of each (lane 0: column i0-1; lane 63: column i0+W).
+template <typename T, int V>
+struct RowData {
+    T h[V], z[V], qx[V], qy[V];
+    T hH, zH, qyH, qxH;
+};
+
+// Per-tile constants of one step.
+template <typename T>
+struct TileCtx {
+    gptr<const T> z, Ho, Qxo, Qyo;
+    gptr<T> Hn, Qxn, Qyn;
+    long long P;
+    int nx, ny, i0, j1, c0, rc, rf, lane;
+    int hcol_c, hcol_f;
+    bool hcell, hface;
+    T nod, sc, dxT, gT, dtdA, rainT;
+    bool rain_on, report_due;
+    double t, dt, dA;
+    FluxConstsF64 kd;
+    FluxConstsF32 kf;
+};
+
+// Per-tile values carried from row to row.  The point-list cursors live in
+// scalar registers: they are uniform, and a vector-memory value here would
+// make every row wait for all loads in flight.
+template <typename T, int V>
+struct TileState {
+    T qyN[V];
+    int sp, se, src_next_j, op, oe, ovr_next_j;
+    T hmax;
+    double vol_s, vol_c;
+    unsigned long long wet;
+};
+
+template <typename T, int V, bool F64, int PF = 1>
 struct Stepper {
     static constexpr int W = 64 * V;
 
-    // Interior face flux; q values are already scaled to q_old units.
-    static __device__ __forceinline__ T flux(T z0, T z1, T h0, T h1, T q0, T qt, T nod,
-                                             const FluxConstsF64& kd, const FluxConstsF32& kf) {
+    // Interior face flux from stored discharges: Q0 of the face and the four
+    // Q of its transverse average (kernels.cpp:79, 129).
+    static __device__ __forceinline__ T flux(T z0, T z1, T h0, T h1, T Q0, T a, T b, T c, T d,
+                                             T nod, T sc, const FluxConstsF64& kd,
+                                             const FluxConstsF32& kf) {
         if (z0 == nod || z1 == nod) return T(0);
         if constexpr (F64) {
-            return face_flux_ref_t1(z0, z1, h0, h1, q0, qt, kd);
+            return face_flux_ref_t1_q(z0, z1, h0, h1, Q0, a, b, c, d, sc, kd);
         } else {
-            return face_flux_f32(z0, z1, h0, h1, q0, qt, kf);
+            return face_flux_f32_q(z0, z1, h0, h1, Q0, a, b, c, d, kf);
         }
     }
 
+    // Row r of H, z, Qx (cell rows, r < rc) and Qy (face rows, r < rf).
+    static __device__ __forceinline__ void load(RowData<T, V>& d, const TileCtx<T>& x, int r) {
+        load_row<T, V>(d.h, x.Ho, x.P, r, x.rc, x.c0);
+        load_row<T, V>(d.z, x.z, x.P, r, x.rc, x.c0);
+        load_row<T, V>(d.qx, x.Qxo, x.P, r, x.rc, x.c0);
+        load_row<T, V>(d.qy, x.Qyo, x.P, r, x.rf, x.c0);
+        d.hH = load_one(x.Ho, x.P, r, x.rc, x.hcol_c);
+        d.zH = load_one(x.z, x.P, r, x.rc, x.hcol_c);
+        d.qyH = load_one(x.Qyo, x.P, r, x.rf, x.hcol_c);
+        d.qxH = load_one(x.Qxo, x.P, r, x.rc, x.hcol_f);
+    }
+
+    // Advances row j of the tile: c holds row j, s row j+1, and row j+1+PF
+    // is loaded into nw before any of this row's work so that PF rows of
+    // loads are in flight while the row computes.
+    template <bool EDGE>
+    static __device__ __forceinline__ void row(const ScnDev& S, const TileCtx<T>& x,
+                                               TileState<T, V>& st, const RowData<T, V>& c,
+                                               const RowData<T, V>& s, RowData<T, V>& nw, int j) {
+        load(nw, x, j + 1 + PF);
+        const int lane = x.lane, c0 = x.c0, nx = x.nx;
+        const T nod = x.nod, sc = x.sc;
+
+        // neighbours across lanes for rows j and j+1
+        const T h_prev = lane_prev(c.h[V - 1], c.hH, lane);
+        const T z_prev = lane_prev(c.z[V - 1], c.zH, lane);
+        const T qyc_prev = lane_prev(c.qy[V - 1], c.qyH, lane);
+        const T qys_prev = lane_prev(s.qy[V - 1], s.qyH, lane);
+        const T qxc_next = lane_next(c.qx[0], c.qxH, lane);
+        const T qxs_next = lane_next(s.qx[0], s.qxH, lane);
+
+        // x faces f = c0+v (west face of column c0+v)
+        T qxw[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const int f = c0 + v;
+            T zl = (v == 0) ? z_prev : c.z[v - 1];
+            T hl = (v == 0) ? h_prev : c.h[v - 1];
+            T ql0 = (v == 0) ? qyc_prev : c.qy[v - 1];
+            T ql1 = (v == 0) ? qys_prev : s.qy[v - 1];
+            T q;
+            if (!EDGE || (f > 0 && f < nx)) {
+                q = flux(zl, c.z[v], hl, c.h[v], c.qx[v], ql0, c.qy[v], ql1, s.qy[v], nod, sc, x.kd,
+                         x.kf);
+            } else if (f == 0) {
+                q = S.west_free
+                        ? edge_west<T>(c.z[0], c.h[0], c.z[1], c.h[1], c.z[0] == nod, x.dxT, x.gT)
+                        : T(0);
+            } else if (f == nx) {
+                q = S.east_free ? edge_out_pos<T>(hl, zl == nod, x.dxT, x.gT) : T(0);
+            } else {
+                q = T(0);
+            }
+            qxw[v] = q;
+        }
+        // east face of the lane's last column: the next lane's west face,
+        // or for lane 63 the strip's east face
+        T qxe_last = __shfl_down(qxw[0], 1, 64);
+        if (lane == 63) {
+            const int f = x.i0 + W;
+            T q = T(0);
+            if (!EDGE || f < nx) {
+                q = flux(c.z[V - 1], c.zH, c.h[V - 1], c.hH, c.qxH, c.qy[V - 1], c.qyH, s.qy[V - 1],
+                         s.qyH, nod, sc, x.kd, x.kf);
+            } else if (f == nx) {
+                q = S.east_free ? edge_out_pos<T>(c.h[V - 1], c.z[V - 1] == nod, x.dxT, x.gT)
+                                : T(0);
+            }
+            qxe_last = q;
+        }
+
+        // y faces south of row j (face row j+1)
+        T qyS[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            T q = T(0);
+            if (!EDGE || c0 + v < nx) {
+                if (j + 1 < x.ny) {
+                    T c1 = (v + 1 < V) ? c.qx[v + 1] : qxc_next;
+                    T s1 = (v + 1 < V) ? s.qx[v + 1] : qxs_next;
+                    q = flux(c.z[v], s.z[v], c.h[v], s.h[v], s.qy[v], c.qx[v], c1, s.qx[v], s1, nod,
+                             sc, x.kd, x.kf);
+                } else if (S.south_free) {
+                    q = edge_out_pos<T>(c.h[v], c.z[v] == nod, x.dxT, x.gT);
+                }
+            }
+            qyS[v] = q;
+        }
+
+        // cell update of row j
+        T hn[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) hn[v] = c.h[v];
+        // point sources of this row, in (column, file) order (main.cpp:245-261)
+        while (__builtin_expect(st.src_next_j == j, 0)) apply_source(S, x, st, hn);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            T h = hn[v];
+            if (x.rain_on && c.z[v] != nod) {  // kernels.cpp:361-364
+                h = h + x.rainT;
+                if (h < T(0)) h = T(0);
+            }
+            const T qe = (v + 1 < V) ? qxw[v + 1] : qxe_last;
+            const T div = ((qxw[v] - qe) + st.qyN[v]) - qyS[v];
+            if constexpr (F64) h = h + (T)(x.dt * (double)div / x.dA);  // kernels.cpp:286
+            else h = h + x.dtdA * div;
+            if (h < T(0)) h = T(0);
+            hn[v] = h;
+        }
+        // FREE / HFIX / HVAR overrides, last in file order wins
+        while (__builtin_expect(st.ovr_next_j == j, 0)) apply_override(S, x, st, hn);
+        // H maximum of the new state over valid cells, mass-report sums
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (!EDGE || c0 + v < nx) {
+                if (c.z[v] != nod && hn[v] > st.hmax) st.hmax = hn[v];
+                if (x.report_due) {
+                    const double hd = (double)hn[v];
+                    kahan_add(st.vol_s, st.vol_c, hd * x.dA);
+                    if (hd > 0.01) st.wet++;
+                }
+            }
+        }
+
+        // stores: H row j, Qx faces of row j, Qy face row j+1
+        if (!EDGE || c0 < x.P) {
+            Vec<T, V> oh, ox, oy;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                oh[v] = (!EDGE || c0 + v < nx) ? hn[v] : T(0);
+                ox[v] = (!EDGE || c0 + v <= nx) ? qxw[v] : T(0);
+                oy[v] = (!EDGE || c0 + v < nx) ? qyS[v] : T(0);
+            }
+            const long long rowo = (long long)j * x.P + c0;
+            *(gptr<Vec<T, V>>)(x.Hn + rowo) = oh;
+            *(gptr<Vec<T, V>>)(x.Qxn + rowo) = ox;
+            *(gptr<Vec<T, V>>)(x.Qyn + rowo + x.P) = oy;
+        }
+        if (EDGE && lane == 63 && x.i0 + W == nx) x.Qxn[(long long)j * x.P + nx] = qxe_last;
+#pragma unroll
+        for (int v = 0; v < V; v++) st.qyN[v] = qyS[v];
+    }
+
+    // One QFIX/QVAR source: v at the step's start time, ps = v*dx*dt/dA added
+    // with a clamp (main.cpp:246-260).
+    static __device__ __forceinline__ void apply_source(const ScnDev& S,
+                                                                  const TileCtx<T>& x,
+                                                                  TileState<T, V>& st, T (&hn)[V]) {
+        const int sp = st.sp;
+        const int col = glob<const int>(S.src_i)[sp];
+        const int pt = glob<const int>(S.src_pt)[sp];
+        gptr<const double> pt_value = glob<const double>(S.pt_value);
+        gptr<const int> pt_off = glob<const int>(S.pt_off);
+        double val;
+        if (glob<const int>(S.pt_type)[pt] == 4) val = pt_value[pt];
+        else val = interp_series(glob<const double>(S.series_t) + pt_off[pt],
+                                 glob<const double>(S.series_v) + pt_off[pt],
+                                 glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+        const double ps = val * S.dx * x.dt / x.dA;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (col == x.c0 + v) {
+                T h = hn[v] + (T)ps;
+                if (h < T(0)) h = T(0);
+                hn[v] = h;
+            }
+        }
+        st.sp = sp + 1;
+        st.src_next_j = __builtin_amdgcn_readfirstlane(
+            (sp + 1 < st.se) ? glob<const int>(S.src_j)[sp + 1] : INT_MAX);
+    }
+
+    // One FREE/HFIX/HVAR point: H = 0, the fixed depth, or the depth series
+    // at the step's start time (kernels.cpp:266-273, main.cpp:275-279).
+    static __device__ __forceinline__ void apply_override(const ScnDev& S,
+                                                                    const TileCtx<T>& x,
+                                                                    TileState<T, V>& st,
+                                                                    T (&hn)[V]) {
+        const int op = st.op;
+        const int col = glob<const int>(S.ovr_i)[op];
+        const int pt = glob<const int>(S.ovr_pt)[op];
+        const int ty = glob<const int>(S.pt_type)[pt];
+        gptr<const double> pt_value = glob<const double>(S.pt_value);
+        gptr<const int> pt_off = glob<const int>(S.pt_off);
+        double val = 0.0;
+        if (ty == 2) val = pt_value[pt];
+        else if (ty == 3)
+            val = interp_series(glob<const double>(S.series_t) + pt_off[pt],
+                                glob<const double>(S.series_v) + pt_off[pt],
+                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            if (col == x.c0 + v) hn[v] = (T)val;
+        }
+        st.op = op + 1;
+        st.ovr_next_j = __builtin_amdgcn_readfirstlane(
+            (op + 1 < st.oe) ? glob<const int>(S.ovr_j)[op + 1] : INT_MAX);
+    }
+
+    // EDGE = false: the strip touches neither the west nor the east edge and
+    // every lane's columns are cells, so no per-face edge test is needed.
+    template <bool EDGE>
     static __device__ __forceinline__ void run(const ScnDev& S, const TileDesc td, int kmod6, int lane) {
         const int cur = kmod6 & 1, nxt = cur ^ 1;
         const int hs_cur = kmod6 % 3, hs_nxt = (kmod6 + 1) % 3, hs_rst = (kmod6 + 2) % 3;
-        Ctl* ctl = S.ctl;
+        gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         const int tile_local = td.strip + td.seg * S.nstrips;
 
         const double t = ctl->t[cur];
@@ -169,296 +421,148 @@ struct Stepper {
                                   report_due, lane);
         if (skip) return;
 
-        const int nx = S.nx, ny = S.ny;
-        const long long P = S.P;
-        const int i0 = td.strip * W;
+        TileCtx<T> x;
+        x.nx = S.nx;
+        x.ny = S.ny;
+        x.P = S.P;
+        x.i0 = td.strip * W;
         const int j0 = td.seg * S.tile_rows;
-        const int j1 = min(j0 + S.tile_rows, ny);
-        const int c0 = i0 + lane * V;
-        const T nod = (T)S.nodata_t;
-        const T* __restrict__ z = reinterpret_cast<const T*>(S.z);
-        const T* __restrict__ Ho = reinterpret_cast<const T*>(S.H[cur]);
-        T* __restrict__ Hn = reinterpret_cast<T*>(S.H[nxt]);
-        const T* __restrict__ Qxo = reinterpret_cast<const T*>(S.Qx[cur]);
-        T* __restrict__ Qxn = reinterpret_cast<T*>(S.Qx[nxt]);
-        const T* __restrict__ Qyo = reinterpret_cast<const T*>(S.Qy[cur]);
-        T* __restrict__ Qyn = reinterpret_cast<T*>(S.Qy[nxt]);
-        const T sc = (T)S.dxinv;  // Q -> q_old (kernels.cpp:8-16)
-        const T dxT = (T)S.dx;
-        const T gT = (T)S.g;
-
-        FluxConstsF64 kd{S.dx, S.fn, S.thresh, S.maxh, S.g * dt, S.theta};
-        FluxConstsF32 kf{(float)S.dx, (float)S.thresh, (float)S.maxh,
-                         (float)(S.g * dt / S.dx), (float)(S.g * dt * S.fn * S.fn)};
-        const double dA = S.dA;
-        const T dtdA = (T)(dt / dA);
-
+        x.j1 = min(j0 + S.tile_rows, x.ny);
+        x.c0 = x.i0 + lane * V;
+        x.lane = lane;
+        x.nod = (T)S.nodata_t;
+        x.z = glob<const T>(S.z);
+        x.Ho = glob<const T>(S.H[cur]);
+        x.Hn = glob<T>(S.H[nxt]);
+        x.Qxo = glob<const T>(S.Qx[cur]);
+        x.Qxn = glob<T>(S.Qx[nxt]);
+        x.Qyo = glob<const T>(S.Qy[cur]);
+        x.Qyn = glob<T>(S.Qy[nxt]);
+        x.sc = (T)S.dxinv;  // Q -> q_old (kernels.cpp:8-16)
+        x.dxT = (T)S.dx;
+        x.gT = (T)S.g;
+        x.kd = FluxConstsF64{S.dx, S.fn, S.thresh, S.maxh, S.g * dt, S.theta};
+        x.kf = FluxConstsF32{(float)S.dx, (float)S.thresh, (float)S.maxh,
+                             (float)(S.g * dt / S.dx), (float)(S.g * dt * S.fn * S.fn),
+                             (float)S.dxinv, (float)(0.25 * S.dxinv)};
+        x.t = t;
+        x.dt = dt;
+        x.dA = S.dA;
+        x.dtdA = (T)(dt / S.dA);
+        x.report_due = report_due;
         // rain depth of this step (main.cpp:265-268, kernels.cpp:356-358)
         double rain_depth = 0.0;
-        bool rain_on = false;
+        x.rain_on = false;
         if (S.rain_enabled) {
-            double r = interp_series(S.rain_t, S.rain_r, S.n_rain, 0.0, t);
-            rain_on = (r > 0.0);
-            if (rain_on) rain_depth = r * dt;
+            double r = interp_series(glob<const double>(S.rain_t), glob<const double>(S.rain_r),
+                                     S.n_rain, 0.0, t);
+            x.rain_on = (r > 0.0);
+            if (x.rain_on) rain_depth = r * dt;
         }
-        const T rainT = (T)rain_depth;
-
+        x.rainT = (T)rain_depth;
         // halo columns: lane 0 reads column i0-1, every other lane column i0+W
         // (only lane 63 uses it; the others hit the same cache line)
-        const int hcol = (lane == 0) ? i0 - 1 : i0 + W;
-        const bool hcell = (lane == 0) ? (i0 > 0) : (i0 + W < nx);   // a cell column
-        const bool hface = (lane == 0) ? false : (i0 + W <= nx);     // an x-face column
-        const int hcol_c = hcell ? hcol : 0;
-        const int hcol_f = hface ? hcol : 0;
+        const int hcol = (lane == 0) ? x.i0 - 1 : x.i0 + W;
+        x.hcell = !EDGE || ((lane == 0) ? (x.i0 > 0) : (x.i0 + W < x.nx));  // a cell column
+        x.hface = (lane != 0) && (!EDGE || x.i0 + W <= x.nx);              // an x-face column
+        x.hcol_c = x.hcell ? hcol : 0;
+        x.hcol_f = x.hface ? hcol : 0;
+        // rows are needed up to j1 (the south neighbour of the tile's last row)
+        x.rc = min(x.ny, x.j1 + 1);
+        x.rf = min(x.ny + 1, x.j1 + 1);
 
-        // sliding window: c = row j, s = row j+1, p = row j+2 (prefetch)
-        T hc[V], zc[V], qxc[V], qyc[V];
-        T hs[V], zs[V], qxs[V], qys[V];
-        T hH_c, zH_c, qyH_c, qxH_c;  // halo values for row j
-        T hH_s, zH_s, qyH_s, qxH_s;  // halo values for row j+1
-
-        load_row<T, V>(hc, Ho, P, j0, ny, c0);
-        load_row<T, V>(zc, z, P, j0, ny, c0);
-        load_row<T, V>(qxc, Qxo, P, j0, ny, c0);
-        load_row<T, V>(qyc, Qyo, P, j0, ny + 1, c0);
-        hH_c = load_one(Ho, P, j0, ny, hcol_c, hcell);
-        zH_c = load_one(z, P, j0, ny, hcol_c, hcell);
-        qyH_c = load_one(Qyo, P, j0, ny + 1, hcol_c, hcell);
-        qxH_c = load_one(Qxo, P, j0, ny, hcol_f, hface);
-        load_row<T, V>(hs, Ho, P, j0 + 1, ny, c0);
-        load_row<T, V>(zs, z, P, j0 + 1, ny, c0);
-        load_row<T, V>(qxs, Qxo, P, j0 + 1, ny, c0);
-        load_row<T, V>(qys, Qyo, P, j0 + 1, ny + 1, c0);
-        hH_s = load_one(Ho, P, j0 + 1, ny, hcol_c, hcell);
-        zH_s = load_one(z, P, j0 + 1, ny, hcol_c, hcell);
-        qyH_s = load_one(Qyo, P, j0 + 1, ny + 1, hcol_c, hcell);
-        qxH_s = load_one(Qxo, P, j0 + 1, ny, hcol_f, hface);
+        TileState<T, V> st;
+        RowData<T, V> A, B, C, D;
+        load(A, x, j0);
+        load(B, x, j0 + 1);
+        if constexpr (PF == 2) load(C, x, j0 + 2);
 
         // north faces of row j0
-        T qyN[V];
         if (j0 == 0) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 T q = T(0);
-                if (S.north_free && c0 + v < nx) q = edge_out_neg<T>(hc[v], zc[v] == nod, dxT, gT);
-                qyN[v] = q;
+                if (S.north_free && (!EDGE || x.c0 + v < x.nx))
+                    q = edge_out_neg<T>(A.h[v], A.z[v] == x.nod, x.dxT, x.gT);
+                st.qyN[v] = q;
             }
-            if (c0 < P) {
-                Pack<T, V> o;
+            if (!EDGE || x.c0 < x.P) {
+                Vec<T, V> o;
 #pragma unroll
-                for (int v = 0; v < V; v++) o.v[v] = (c0 + v < nx) ? qyN[v] : T(0);
-                *reinterpret_cast<Pack<T, V>*>(Qyn + c0) = o;
+                for (int v = 0; v < V; v++) o[v] = (!EDGE || x.c0 + v < x.nx) ? st.qyN[v] : T(0);
+                *(gptr<Vec<T, V>>)(x.Qyn + x.c0) = o;
             }
         } else {
             T ha[V], za[V], qxa[V];
-            load_row<T, V>(ha, Ho, P, j0 - 1, ny, c0);
-            load_row<T, V>(za, z, P, j0 - 1, ny, c0);
-            load_row<T, V>(qxa, Qxo, P, j0 - 1, ny, c0);
-            T qxH_a = load_one(Qxo, P, j0 - 1, ny, hcol_f, hface);
+            load_row<T, V>(ha, x.Ho, x.P, j0 - 1, x.ny, x.c0);
+            load_row<T, V>(za, x.z, x.P, j0 - 1, x.ny, x.c0);
+            load_row<T, V>(qxa, x.Qxo, x.P, j0 - 1, x.ny, x.c0);
+            T qxH_a = load_one(x.Qxo, x.P, j0 - 1, x.ny, x.hcol_f);
             T qxa_n = lane_next(qxa[0], qxH_a, lane);
-            T qxc_n = lane_next(qxc[0], qxH_c, lane);
+            T qxc_n = lane_next(A.qx[0], A.qxH, lane);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 T a1 = (v + 1 < V) ? qxa[v + 1] : qxa_n;
-                T c1 = (v + 1 < V) ? qxc[v + 1] : qxc_n;
-                T qt = (((qxa[v] * sc + a1 * sc) + qxc[v] * sc) + c1 * sc) * T(0.25);
-                qyN[v] = (c0 + v < nx) ? flux(za[v], zc[v], ha[v], hc[v], qyc[v] * sc, qt, nod, kd, kf)
-                                       : T(0);
+                T c1 = (v + 1 < V) ? A.qx[v + 1] : qxc_n;
+                st.qyN[v] = (!EDGE || x.c0 + v < x.nx)
+                                ? flux(za[v], A.z[v], ha[v], A.h[v], A.qy[v], qxa[v], a1, A.qx[v], c1,
+                                       x.nod, x.sc, x.kd, x.kf)
+                                : T(0);
             }
         }
 
         // per-tile point lists (uniform)
-        int sp = S.src_begin[tile_local];
-        const int se = S.src_begin[tile_local + 1];
-        int src_next_j = (sp < se) ? S.src_j[sp] : INT_MAX;
-        int op = S.ovr_begin[tile_local];
-        const int oe = S.ovr_begin[tile_local + 1];
-        int ovr_next_j = (op < oe) ? S.ovr_j[op] : INT_MAX;
+        gptr<const int> src_begin = glob<const int>(S.src_begin);
+        gptr<const int> ovr_begin = glob<const int>(S.ovr_begin);
+        st.sp = __builtin_amdgcn_readfirstlane(src_begin[tile_local]);
+        st.se = __builtin_amdgcn_readfirstlane(src_begin[tile_local + 1]);
+        st.src_next_j = __builtin_amdgcn_readfirstlane(
+            (st.sp < st.se) ? glob<const int>(S.src_j)[st.sp] : INT_MAX);
+        st.op = __builtin_amdgcn_readfirstlane(ovr_begin[tile_local]);
+        st.oe = __builtin_amdgcn_readfirstlane(ovr_begin[tile_local + 1]);
+        st.ovr_next_j = __builtin_amdgcn_readfirstlane(
+            (st.op < st.oe) ? glob<const int>(S.ovr_j)[st.op] : INT_MAX);
+        st.hmax = F64 ? T(-DBL_MAX) : T(-FLT_MAX);
+        st.vol_s = 0.0;
+        st.vol_c = 0.0;
+        st.wet = 0;
 
-        T hmax_l = F64 ? T(-DBL_MAX) : T(-FLT_MAX);
-        double vol_s = 0.0, vol_c = 0.0;
-        unsigned long long wet = 0;
-
+        // PF+2 register sets rotate through the roles (row j, row j+1, rows
+        // in flight); unrolling by PF+2 keeps every role in fixed registers
+        const int j1 = x.j1;
+        if constexpr (PF == 1) {
 #pragma unroll 1
-        for (int j = j0; j < j1; j++) {
-            // prefetch row j+2
-            T hp[V], zp[V], qxp[V], qyp[V];
-            load_row<T, V>(hp, Ho, P, j + 2, ny, c0);
-            load_row<T, V>(zp, z, P, j + 2, ny, c0);
-            load_row<T, V>(qxp, Qxo, P, j + 2, ny, c0);
-            load_row<T, V>(qyp, Qyo, P, j + 2, ny + 1, c0);
-            T hH_p = load_one(Ho, P, j + 2, ny, hcol_c, hcell);
-            T zH_p = load_one(z, P, j + 2, ny, hcol_c, hcell);
-            T qyH_p = load_one(Qyo, P, j + 2, ny + 1, hcol_c, hcell);
-            T qxH_p = load_one(Qxo, P, j + 2, ny, hcol_f, hface);
-
-            // neighbours across lanes for rows j and j+1
-            const T h_prev = lane_prev(hc[V - 1], hH_c, lane);
-            const T z_prev = lane_prev(zc[V - 1], zH_c, lane);
-            const T qyc_prev = lane_prev(qyc[V - 1], qyH_c, lane);
-            const T qys_prev = lane_prev(qys[V - 1], qyH_s, lane);
-            const T qxc_next = lane_next(qxc[0], qxH_c, lane);
-            const T qxs_next = lane_next(qxs[0], qxH_s, lane);
-
-            // x faces f = c0+v (west face of column c0+v)
-            T qxw[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                const int f = c0 + v;
-                T zl = (v == 0) ? z_prev : zc[v - 1];
-                T hl = (v == 0) ? h_prev : hc[v - 1];
-                T ql0 = (v == 0) ? qyc_prev : qyc[v - 1];
-                T ql1 = (v == 0) ? qys_prev : qys[v - 1];
-                T q;
-                if (f > 0 && f < nx) {
-                    T qt = (((ql0 * sc + qyc[v] * sc) + ql1 * sc) + qys[v] * sc) * T(0.25);
-                    q = flux(zl, zc[v], hl, hc[v], qxc[v] * sc, qt, nod, kd, kf);
-                } else if (f == 0) {
-                    q = S.west_free ? edge_west<T>(zc[0], hc[0], zc[1], hc[1], zc[0] == nod, dxT, gT)
-                                    : T(0);
-                } else if (f == nx) {
-                    q = S.east_free ? edge_out_pos<T>(hl, zl == nod, dxT, gT) : T(0);
-                } else {
-                    q = T(0);
-                }
-                qxw[v] = q;
+            for (int j = j0; j < j1; j += 3) {
+                row<EDGE>(S, x, st, A, B, C, j);
+                if (j + 1 >= j1) break;
+                row<EDGE>(S, x, st, B, C, A, j + 1);
+                if (j + 2 >= j1) break;
+                row<EDGE>(S, x, st, C, A, B, j + 2);
             }
-            // east face of the lane's last column: the next lane's west face,
-            // or for lane 63 the strip's east face
-            T qxe_last = __shfl_down(qxw[0], 1, 64);
-            if (lane == 63) {
-                const int f = i0 + W;
-                T q = T(0);
-                if (f < nx) {
-                    T qt = (((qyc[V - 1] * sc + qyH_c * sc) + qys[V - 1] * sc) + qyH_s * sc) * T(0.25);
-                    q = flux(zc[V - 1], zH_c, hc[V - 1], hH_c, qxH_c * sc, qt, nod, kd, kf);
-                } else if (f == nx) {
-                    q = S.east_free ? edge_out_pos<T>(hc[V - 1], zc[V - 1] == nod, dxT, gT) : T(0);
-                }
-                qxe_last = q;
+        } else {
+#pragma unroll 1
+            for (int j = j0; j < j1; j += 4) {
+                row<EDGE>(S, x, st, A, B, D, j);
+                if (j + 1 >= j1) break;
+                row<EDGE>(S, x, st, B, C, A, j + 1);
+                if (j + 2 >= j1) break;
+                row<EDGE>(S, x, st, C, D, B, j + 2);
+                if (j + 3 >= j1) break;
+                row<EDGE>(S, x, st, D, A, C, j + 3);
             }
-
-            // y faces south of row j (face row j+1)
-            T qyS[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                T q = T(0);
-                if (c0 + v < nx) {
-                    if (j + 1 < ny) {
-                        T c1 = (v + 1 < V) ? qxc[v + 1] : qxc_next;
-                        T s1 = (v + 1 < V) ? qxs[v + 1] : qxs_next;
-                        T qt = (((qxc[v] * sc + c1 * sc) + qxs[v] * sc) + s1 * sc) * T(0.25);
-                        q = flux(zc[v], zs[v], hc[v], hs[v], qys[v] * sc, qt, nod, kd, kf);
-                    } else if (S.south_free) {
-                        q = edge_out_pos<T>(hc[v], zc[v] == nod, dxT, gT);
-                    }
-                }
-                qyS[v] = q;
-            }
-
-            // cell update of row j
-            T hn[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) hn[v] = hc[v];
-            // point sources of this row, in (column, file) order (main.cpp:245-261)
-            while (src_next_j == j) {
-                const int col = S.src_i[sp];
-                const int pt = S.src_pt[sp];
-                double val;
-                if (S.pt_type[pt] == 4) val = S.pt_value[pt];
-                else val = interp_series(S.series_t + S.pt_off[pt], S.series_v + S.pt_off[pt],
-                                         S.pt_len[pt], S.pt_value[pt], t);
-                const double ps = val * S.dx * dt / dA;
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    if (col == c0 + v) {
-                        T h = hn[v] + (T)ps;
-                        if (h < T(0)) h = T(0);
-                        hn[v] = h;
-                    }
-                }
-                sp++;
-                src_next_j = (sp < se) ? S.src_j[sp] : INT_MAX;
-            }
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                T h = hn[v];
-                if (rain_on && zc[v] != nod) {  // kernels.cpp:361-364
-                    h = h + rainT;
-                    if (h < T(0)) h = T(0);
-                }
-                const T qe = (v + 1 < V) ? qxw[v + 1] : qxe_last;
-                const T div = ((qxw[v] - qe) + qyN[v]) - qyS[v];
-                if constexpr (F64) h = h + (T)(dt * (double)div / dA);  // kernels.cpp:286
-                else h = h + dtdA * div;
-                if (h < T(0)) h = T(0);
-                hn[v] = h;
-            }
-            // FREE / HFIX / HVAR overrides, last in file order wins
-            while (ovr_next_j == j) {
-                const int col = S.ovr_i[op];
-                const int pt = S.ovr_pt[op];
-                const int ty = S.pt_type[pt];
-                double val = 0.0;
-                if (ty == 2) val = S.pt_value[pt];
-                else if (ty == 3)
-                    val = interp_series(S.series_t + S.pt_off[pt], S.series_v + S.pt_off[pt],
-                                        S.pt_len[pt], S.pt_value[pt], t);
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    if (col == c0 + v) hn[v] = (T)val;
-                }
-                op++;
-                ovr_next_j = (op < oe) ? S.ovr_j[op] : INT_MAX;
-            }
-            // H maximum of the new state over valid cells, mass-report sums
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                if (c0 + v < nx) {
-                    if (zc[v] != nod && hn[v] > hmax_l) hmax_l = hn[v];
-                    if (report_due) {
-                        const double hd = (double)hn[v];
-                        kahan_add(vol_s, vol_c, hd * dA);
-                        if (hd > 0.01) wet++;
-                    }
-                }
-            }
-
-            // stores: H row j, Qx faces of row j, Qy face row j+1
-            if (c0 < P) {
-                Pack<T, V> oh, ox, oy;
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    oh.v[v] = (c0 + v < nx) ? hn[v] : T(0);
-                    ox.v[v] = (c0 + v <= nx) ? qxw[v] : T(0);
-                    oy.v[v] = (c0 + v < nx) ? qyS[v] : T(0);
-                }
-                const long long row = (long long)j * P + c0;
-                *reinterpret_cast<Pack<T, V>*>(Hn + row) = oh;
-                *reinterpret_cast<Pack<T, V>*>(Qxn + row) = ox;
-                *reinterpret_cast<Pack<T, V>*>(Qyn + row + P) = oy;
-            }
-            if (lane == 63 && i0 + W == nx) Qxn[(long long)j * P + nx] = qxe_last;
-
-            // slide the window
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                qyN[v] = qyS[v];
-                hc[v] = hs[v]; zc[v] = zs[v]; qxc[v] = qxs[v]; qyc[v] = qys[v];
-                hs[v] = hp[v]; zs[v] = zp[v]; qxs[v] = qxp[v]; qys[v] = qyp[v];
-            }
-            hH_c = hH_s; zH_c = zH_s; qyH_c = qyH_s; qxH_c = qxH_s;
-            hH_s = hH_p; zH_s = zH_p; qyH_s = qyH_p; qxH_s = qxH_p;
         }
 
         // wavefront max, one atomic per wavefront
-        for (int off = 32; off > 0; off >>= 1) hmax_l = getmax(hmax_l, __shfl_xor(hmax_l, off, 64));
+        T hm = st.hmax;
+        for (int off = 32; off > 0; off >>= 1) hm = getmax(hm, __shfl_xor(hm, off, 64));
         if (lane == 0) {
             // the empty-set identity stays the lowest double, as a Kokkos max reducer's
-            const double m = (hmax_l == (F64 ? T(-DBL_MAX) : T(-FLT_MAX))) ? -DBL_MAX : (double)hmax_l;
-            atomicMax(&ctl->hmax[hs_nxt], fkey(m));
+            const double m = (hm == (F64 ? T(-DBL_MAX) : T(-FLT_MAX))) ? -DBL_MAX : (double)hm;
+            atomicMax(&S.ctl->hmax[hs_nxt], fkey(m));
         }
         if (report_due) {
+            double vol_s = st.vol_s, vol_c = st.vol_c;
+            unsigned long long wet = st.wet;
             for (int off = 32; off > 0; off >>= 1) {
                 double os = __shfl_xor(vol_s, off, 64);
                 double oc = __shfl_xor(vol_c, off, 64);
@@ -468,17 +572,17 @@ struct Stepper {
                 double b = (lane & off) ? vol_s : os;
                 double ca = (lane & off) ? oc : vol_c;
                 double cb = (lane & off) ? vol_c : oc;
-                double s = a, c = ca + cb;
-                kahan_add(s, c, b);
-                vol_s = s;
-                vol_c = c;
+                double s2 = a, c2 = ca + cb;
+                kahan_add(s2, c2, b);
+                vol_s = s2;
+                vol_c = c2;
                 wet += ow;
             }
             if (lane == 0) {
                 const long long slot = (long long)cur * S.ntiles + tile_local;
-                S.part_vol[slot] = vol_s;
-                S.part_comp[slot] = vol_c;
-                S.part_area[slot] = wet;
+                glob<double>(S.part_vol)[slot] = vol_s;
+                glob<double>(S.part_comp)[slot] = vol_c;
+                glob<unsigned long long>(S.part_area)[slot] = wet;
             }
         }
     }
@@ -489,7 +593,7 @@ struct Stepper {
     static __device__ __forceinline__ void bookkeeping(const ScnDev& S, int cur, int nxt, int hs_cur, int hs_nxt,
                                        int hs_rst, double t, double dt, double next_mass,
                                        bool skip, bool report_due, int lane) {
-        Ctl* ctl = S.ctl;
+        gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         finalize_report(S, lane);
         if (lane == 0) {
             if (!skip) {
@@ -497,7 +601,7 @@ struct Stepper {
                 ctl->t[nxt] = t_new;
                 ctl->hmax[hs_rst] = key_neg_max();
                 const long long n = ctl->n_done;
-                if (n < S.dt_log_cap) S.dt_log[n] = dt;
+                if (n < S.dt_log_cap) glob<double>(S.dt_log)[n] = dt;
                 ctl->n_done = n + 1;
                 ctl->last_dt = dt;
                 if (report_due) {  // main.cpp:291-304
@@ -518,15 +622,15 @@ struct Stepper {
     }
 
     static __device__ __forceinline__ void finalize_report(const ScnDev& S, int lane) {
-        Ctl* ctl = S.ctl;
+        gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         if (!ctl->pending) return;
         const long long base = (long long)ctl->pending_buf * S.ntiles;
         double s = 0.0, c = 0.0;
         unsigned long long wet = 0;
         for (int i = lane; i < S.ntiles; i += 64) {
-            kahan_add(s, c, S.part_vol[base + i]);
-            c += S.part_comp[base + i];
-            wet += S.part_area[base + i];
+            kahan_add(s, c, glob<const double>(S.part_vol)[base + i]);
+            c += glob<const double>(S.part_comp)[base + i];
+            wet += glob<const unsigned long long>(S.part_area)[base + i];
         }
         for (int off = 32; off > 0; off >>= 1) {
             double os = __shfl_xor(s, off, 64);
@@ -545,7 +649,7 @@ struct Stepper {
         if (lane == 0) {
             const int r = ctl->n_reports;
             if (r < S.report_cap) {
-                double* rec = S.reports + 4LL * r;
+                gptr<double> rec = glob<double>(S.reports) + 4LL * r;
                 rec[0] = ctl->pending_t;
                 rec[1] = ctl->pending_dt;
                 rec[2] = s + c;
@@ -566,8 +670,8 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     return x * q + min(x, r) + k;
 }
 
-template <typename T, int V, bool F64, int WPB>
-__global__ void __launch_bounds__(64 * WPB)
+template <typename T, int V, bool F64, int WPB, int OCC, int PF>
+__global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_step(const ScnDev* __restrict__ scns, const TileDesc* __restrict__ tiles, int ntiles, int kmod6) {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -575,7 +679,12 @@ k_step(const ScnDev* __restrict__ scns, const TileDesc* __restrict__ tiles, int 
     const int tile = lb * WPB + wv;
     if (tile >= ntiles) return;
     const TileDesc td = tiles[tile];
-    Stepper<T, V, F64>::run(scns[td.scn], td, kmod6, lane);
+    const ScnDev& S = scns[td.scn];
+    constexpr int W = 64 * V;
+    if (td.strip > 0 && (td.strip + 1) * W < S.nx)
+        Stepper<T, V, F64, PF>::template run<false>(S, td, kmod6, lane);
+    else
+        Stepper<T, V, F64, PF>::template run<true>(S, td, kmod6, lane);
 }
 
 }  // namespace af

@@ -26,10 +26,11 @@ def _stream(torch, stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
-def make_tuning(tile_rows=0, waves_per_block=0, graph_steps=0, dt_log_cap=0, report_cap=0):
+def make_tuning(tile_rows=0, waves_per_block=0, graph_steps=0, dt_log_cap=0, report_cap=0,
+                variant=0):
     t = Tuning()
     t.tile_rows, t.waves_per_block, t.graph_steps = tile_rows, waves_per_block, graph_steps
-    t.dt_log_cap, t.report_cap = dt_log_cap, report_cap
+    t.dt_log_cap, t.report_cap, t.variant = dt_log_cap, report_cap, variant
     return t
 
 

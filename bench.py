@@ -38,11 +38,12 @@ def _traffic_from_profile(precision):
     return None
 
 
-def measure(scn, precision, steps, warmup, tile_rows, wpb):
+def measure(scn, precision, steps, warmup, tile_rows, wpb, variant=0):
     import torch
     from adaptflood_amd.domain import Domain, make_tuning
     d = Domain(scn, precision=precision,
-               tuning=make_tuning(tile_rows=tile_rows, waves_per_block=wpb, graph_steps=36))
+               tuning=make_tuning(tile_rows=tile_rows, waves_per_block=wpb, graph_steps=36,
+                                  variant=variant))
     torch.cuda.synchronize()
     d.advance(warmup, graph=False)
     torch.cuda.synchronize()
@@ -80,6 +81,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 4 GiB device-to-device copy (the known-size kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -96,6 +100,13 @@ def main():
     print(f"[bench] generated {scn.nx}x{scn.ny} in {time.perf_counter() - t:.1f}s",
           file=sys.stderr, flush=True)
     cells = scn.cells
+    if args.calibrate_copy:
+        a = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     wall, kern_s, info = measure(scn, "fp32", args.steps, args.warmup, args.tile_rows, args.wpb)
     gcups = cells * args.steps / wall / 1e9

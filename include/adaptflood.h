@@ -174,10 +174,11 @@ typedef struct af_domain af_domain; /* opaque, host side */
  * 2 at FP64) and marches down `tile_rows` rows.  0 = library default. */
 typedef struct af_tuning {
     int32_t tile_rows;
-    int32_t waves_per_block; /* 1, 2 or 4 */
+    int32_t waves_per_block; /* 1 or 4 */
     int32_t graph_steps;     /* steps per captured graph (multiple of 6); 0 = default */
     int32_t dt_log_cap;      /* device dt log entries kept (0 = none) */
     int32_t report_cap;      /* device mass-report records kept */
+    int32_t variant;         /* kernel variant 0..2: target occupancy of the step kernel */
 } af_tuning;
 
 /* Device bytes the domain needs for `desc` at `precision`. */
