@@ -432,19 +432,20 @@ int launch_step_v(af_domain* D, int kmod6, hipStream_t st) {
     return AF_OK;
 }
 
-// Kernel variants: rows of loads each wavefront keeps in flight (PF) and the
-// occupancy (waves per SIMD) the register allocation targets.
+// Kernel variants: rows per loop step (PF = 1: one row, the next in flight;
+// PF = 2: two-row load and store bursts) and the occupancy (waves per SIMD)
+// the register allocation targets.
 int launch_step(af_domain* D, int kmod6, hipStream_t st) {
     if (D->precision == AF_FP64) {
         switch (D->variant) {
             case 1: return launch_step_v<double, 2, true, 2, 2>(D, kmod6, st);
-            case 2: return launch_step_v<double, 2, true, 3, 1>(D, kmod6, st);
+            case 2: return launch_step_v<double, 2, true, 1, 2>(D, kmod6, st);
             default: return launch_step_v<double, 2, true, 2, 1>(D, kmod6, st);
         }
     }
     switch (D->variant) {
         case 1: return launch_step_v<float, 4, false, 3, 2>(D, kmod6, st);
-        case 2: return launch_step_v<float, 4, false, 4, 2>(D, kmod6, st);
+        case 2: return launch_step_v<float, 4, false, 2, 2>(D, kmod6, st);
         default: return launch_step_v<float, 4, false, 4, 1>(D, kmod6, st);
     }
 }

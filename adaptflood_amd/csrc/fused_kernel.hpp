@@ -157,6 +157,15 @@ struct RowData {
     T hH, zH, qyH, qxH;
 };
 
+// Results of one row before they are stored: H of the row, the west x-face
+// of each column, the south y-face of each column, and lane 63's strip-edge
+// east face.
+template <typename T, int V>
+struct RowOut {
+    T h[V], qx[V], qy[V];
+    T qxe;
+};
+
 // Per-tile constants of one step.
 template <typename T>
 struct TileCtx {
@@ -214,14 +223,13 @@ struct Stepper {
         d.qxH = load_one(x.Qxo, x.P, r, x.rc, x.hcol_f);
     }
 
-    // Advances row j of the tile: c holds row j, s row j+1, and row j+1+PF
-    // is loaded into nw before any of this row's work so that PF rows of
-    // loads are in flight while the row computes.
+    // Row j of the tile from c (row j) and s (row j+1): faces, sources, rain,
+    // depth update, overrides and the H-max / report folds; results in o.
     template <bool EDGE>
-    static __device__ __forceinline__ void row(const ScnDev& S, const TileCtx<T>& x,
-                                               TileState<T, V>& st, const RowData<T, V>& c,
-                                               const RowData<T, V>& s, RowData<T, V>& nw, int j) {
-        load(nw, x, j + 1 + PF);
+    static __device__ __forceinline__ void compute(const ScnDev& S, const TileCtx<T>& x,
+                                                   TileState<T, V>& st, const RowData<T, V>& c,
+                                                   const RowData<T, V>& s, int j,
+                                                   RowOut<T, V>& o) {
         const int lane = x.lane, c0 = x.c0, nx = x.nx;
         const T nod = x.nod, sc = x.sc;
 
@@ -326,23 +334,64 @@ struct Stepper {
             }
         }
 
-        // stores: H row j, Qx faces of row j, Qy face row j+1
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            o.h[v] = hn[v];
+            o.qx[v] = qxw[v];
+            o.qy[v] = qyS[v];
+            st.qyN[v] = qyS[v];
+        }
+        o.qxe = qxe_last;
+    }
+
+    // Stores row j: H of the row, its x faces and the y faces south of it.
+    template <bool EDGE>
+    static __device__ __forceinline__ void store(const TileCtx<T>& x, const RowOut<T, V>& o,
+                                                 int j) {
+        const int c0 = x.c0, nx = x.nx;
         if (!EDGE || c0 < x.P) {
             Vec<T, V> oh, ox, oy;
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                oh[v] = (!EDGE || c0 + v < nx) ? hn[v] : T(0);
-                ox[v] = (!EDGE || c0 + v <= nx) ? qxw[v] : T(0);
-                oy[v] = (!EDGE || c0 + v < nx) ? qyS[v] : T(0);
+                oh[v] = (!EDGE || c0 + v < nx) ? o.h[v] : T(0);
+                ox[v] = (!EDGE || c0 + v <= nx) ? o.qx[v] : T(0);
+                oy[v] = (!EDGE || c0 + v < nx) ? o.qy[v] : T(0);
             }
             const long long rowo = (long long)j * x.P + c0;
             *(gptr<Vec<T, V>>)(x.Hn + rowo) = oh;
             *(gptr<Vec<T, V>>)(x.Qxn + rowo) = ox;
             *(gptr<Vec<T, V>>)(x.Qyn + rowo + x.P) = oy;
         }
-        if (EDGE && lane == 63 && x.i0 + W == nx) x.Qxn[(long long)j * x.P + nx] = qxe_last;
-#pragma unroll
-        for (int v = 0; v < V; v++) st.qyN[v] = qyS[v];
+        if (EDGE && x.lane == 63 && x.i0 + W == nx) x.Qxn[(long long)j * x.P + nx] = o.qxe;
+    }
+
+    // One row per call: row j+2 is loaded into nw first, so that one row of
+    // loads is in flight while row j computes.
+    template <bool EDGE>
+    static __device__ __forceinline__ void row(const ScnDev& S, const TileCtx<T>& x,
+                                               TileState<T, V>& st, const RowData<T, V>& c,
+                                               const RowData<T, V>& s, RowData<T, V>& nw, int j) {
+        load(nw, x, j + 2);
+        RowOut<T, V> o;
+        compute<EDGE>(S, x, st, c, s, j, o);
+        store<EDGE>(x, o, j);
+    }
+
+    // Two rows per call: rows j+2 and j+3 are loaded as one burst, rows j and
+    // j+1 computed, and their six stores issued together.
+    template <bool EDGE>
+    static __device__ __forceinline__ void row2(const ScnDev& S, const TileCtx<T>& x,
+                                                TileState<T, V>& st, const RowData<T, V>& a,
+                                                const RowData<T, V>& b, RowData<T, V>& c,
+                                                RowData<T, V>& d, int j) {
+        load(c, x, j + 2);
+        load(d, x, j + 3);
+        RowOut<T, V> o0, o1;
+        compute<EDGE>(S, x, st, a, b, j, o0);
+        const bool two = (j + 1 < x.j1);
+        if (two) compute<EDGE>(S, x, st, b, c, j + 1, o1);
+        store<EDGE>(x, o0, j);
+        if (two) store<EDGE>(x, o1, j + 1);
     }
 
     // One QFIX/QVAR source: v at the step's start time, ps = v*dx*dt/dA added
@@ -475,7 +524,6 @@ struct Stepper {
         RowData<T, V> A, B, C, D;
         load(A, x, j0);
         load(B, x, j0 + 1);
-        if constexpr (PF == 2) load(C, x, j0 + 2);
 
         // north faces of row j0
         if (j0 == 0) {
@@ -527,10 +575,10 @@ struct Stepper {
         st.vol_c = 0.0;
         st.wet = 0;
 
-        // PF+2 register sets rotate through the roles (row j, row j+1, rows
-        // in flight); unrolling by PF+2 keeps every role in fixed registers
         const int j1 = x.j1;
         if constexpr (PF == 1) {
+            // three register sets rotate through the roles (row j, row j+1,
+            // row j+2 in flight); unrolling by three keeps each in fixed registers
 #pragma unroll 1
             for (int j = j0; j < j1; j += 3) {
                 row<EDGE>(S, x, st, A, B, C, j);
@@ -540,15 +588,13 @@ struct Stepper {
                 row<EDGE>(S, x, st, C, A, B, j + 2);
             }
         } else {
+            // two rows per step of the loop; four register sets alternate
+            // between resident and in flight
 #pragma unroll 1
             for (int j = j0; j < j1; j += 4) {
-                row<EDGE>(S, x, st, A, B, D, j);
-                if (j + 1 >= j1) break;
-                row<EDGE>(S, x, st, B, C, A, j + 1);
+                row2<EDGE>(S, x, st, A, B, C, D, j);
                 if (j + 2 >= j1) break;
-                row<EDGE>(S, x, st, C, D, B, j + 2);
-                if (j + 3 >= j1) break;
-                row<EDGE>(S, x, st, D, A, C, j + 3);
+                row2<EDGE>(S, x, st, C, D, A, B, j + 2);
             }
         }
 

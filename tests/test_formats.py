@@ -1,0 +1,215 @@
+"""Known-answer tests for the input readers (C++, through the C ABI).
+
+Each expected value is derived by hand from the reference's reader code:
+read_par src/io.cpp:9-47, read_dem :132-161, read_bci :164-254,
+read_bdy :257-296, read_rain :103-129, interpolation :299-311, write_asc
+:314-334, and the input sequence of src/main.cpp:24-92.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from adaptflood_amd import formats, generators
+from adaptflood_amd._lib import (AF_BC_FREE, AF_BC_HFIX, AF_BC_HVAR, AF_BC_QFIX, AF_BC_QVAR,
+                                 AFError)
+
+
+def _w(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_par_code_keys_and_readme_keys(tmp_path):
+    p = _w(tmp_path, "a.par", "\n".join([
+        "# comment in column 0",
+        "DEMfile   my.dem.ascii",
+        "bcifile my.bci",
+        "bdyfile my.bdy",
+        "dirroot out",
+        "sim_time 7200",
+        "saveint 300",
+        "massint 30",
+        "initial_tstep 5",
+        "cfl 0.5",
+        "fpfric 0.05",
+        "depth_thresh 0.002",
+        "rainfile r.txt",
+        "output_format 1",
+        "voutput",
+        "async_output 0",
+        # README spellings the code ignores
+        "resroot ignored",
+        "fpn 0.9",
+        "depththresh 0.9",
+        "acceleration",
+        "  # indented: not a comment, the key '#' is unknown",
+        "",
+    ]))
+    d = formats.read_par(p)
+    assert d["dem_file"] == "my.dem.ascii" and d["bci_file"] == "my.bci"
+    assert d["output_dir"] == "out" and d["rain_file"] == "r.txt"
+    assert d["sim_time"] == 7200 and d["save_int"] == 300 and d["mass_int"] == 30
+    assert d["init_tstep"] == 5 and d["cfl"] == 0.5
+    assert d["fpn"] == 0.05 and d["depth_thresh"] == 0.002   # not the README's fpn/depththresh
+    assert d["output_format"] == 1 and d["voutput"] == 1 and d["async_output"] == 0
+
+
+def test_par_defaults_and_errors(tmp_path):
+    d = formats.par_defaults()
+    assert d["sim_time"] == 3600 and d["cfl"] == 0.7 and d["fpn"] == 0.035
+    assert d["depth_thresh"] == 0.001 and d["init_tstep"] == 10 and d["mass_int"] == 60
+    assert d["async_output"] == 1 and d["dem_file"] == "dem.ascii"
+    with pytest.raises(AFError):
+        formats.read_par(_w(tmp_path, "b.par", "sim_time\n"))      # std::stod("") throws
+    with pytest.raises(AFError):
+        formats.read_par(str(tmp_path / "missing.par"))
+
+
+def test_dem_positional_header_and_rows(tmp_path):
+    p = _w(tmp_path, "d.asc", "a 3\nb 2\nc 100.0\nd 200.0\ne 10.0\nf -1\n"
+                              "1 2 3\n4 -1 6\n")
+    c = formats.Case()
+    c.read_dem(p)
+    h = c.header()
+    assert h == {"ncols": 3, "nrows": 2, "xll": 100.0, "yll": 200.0, "cellsize": 10.0,
+                 "nodata": -1.0}
+    s = c.scenario()
+    assert np.array_equal(s.dem, [[1, 2, 3], [4, -1, 6]])   # row 0 is the first (north) row
+    # a short file leaves the missing values at 0
+    p2 = _w(tmp_path, "e.asc", "ncols 2\nnrows 2\nx 0\ny 0\nc 1\nn -9999\n5 6 7\n")
+    c2 = formats.Case()
+    c2.read_dem(p2)
+    assert np.array_equal(c2.scenario().dem, [[5, 6], [7, 0]])
+
+
+def _case_with_dem(tmp_path, nx=10, ny=8, cs=10.0, xll=1000.0, yll=2000.0):
+    rows = "\n".join(" ".join("1.0" for _ in range(nx)) for _ in range(ny))
+    p = _w(tmp_path, "g.asc", f"ncols {nx}\nnrows {ny}\nxllcorner {xll}\nyllcorner {yll}\n"
+                              f"cellsize {cs}\nNODATA_value -9999\n{rows}\n")
+    c = formats.Case()
+    c.read_dem(p)
+    return c
+
+
+def test_bci_points_world_coordinates_and_types(tmp_path):
+    c = _case_with_dem(tmp_path)
+    # tly = 2000 + 8*10 = 2080; gx = int((x-1000)/10); gy = int((2080-y)/10)
+    p = _w(tmp_path, "b.bci", "\n".join([
+        "# header",
+        "P 1015 2075 QFIX 0.25",       # (1, 0)
+        "P 1099.9 2000.1 QVAR inflow", # (9, 7)
+        "P 1000 2080 HFIX 1.5",        # (0, 0)
+        "P 1035 2055 HVAR stage",      # (3, 2)
+        "P 1045 2045 FREE",            # (4, 3)
+        "P 1045 2045 QVAR",            # QVAR without a name: skipped
+        "P 1045 2045 WEIR 2",          # unknown type: skipped
+        "P 995 2075 QFIX 1",           # x = -0.5 truncates toward zero: (0, 0)
+        "",
+    ]) + "\n")
+    c.read_bci(p)
+    pts = c.points()
+    got = [(q.x, q.y, q.type, q.value, q.name) for q in pts]
+    assert got == [(1, 0, AF_BC_QFIX, 0.25, ""), (9, 7, AF_BC_QVAR, 0.0, "inflow"),
+                   (0, 0, AF_BC_HFIX, 1.5, ""), (3, 2, AF_BC_HVAR, 0.0, "stage"),
+                   (4, 3, AF_BC_FREE, 0.0, ""), (0, 0, AF_BC_QFIX, 1.0, "")]
+
+
+def test_bci_line_boundaries(tmp_path):
+    c = _case_with_dem(tmp_path)
+    p = _w(tmp_path, "l.bci", "W 0 7 FREE\nE FREE\nN 0 9 HFIX\nS 3 5 FREE\n")
+    c.read_bci(p)
+    # "S a b FREE" works (a, b ignored); "E FREE" is the dead second form; line HFIX stays closed
+    assert c.line_bcs() == "WS"
+    s = c.scenario()
+    assert s.west_free and s.south_free and not s.east_free and not s.north_free
+
+
+def test_bdy_sections_prefix_and_first_match(tmp_path):
+    c = _case_with_dem(tmp_path)
+    c.read_bci(_w(tmp_path, "q.bci", "P 1015 2075 QVAR in1\nP 1025 2075 QVAR in10\n"
+                                     "P 1035 2075 QVAR in1\nP 1045 2075 HVAR lvl\n"))
+    c.read_bdy(_w(tmp_path, "q.bdy", "\n".join([
+        "in1 is skipped on the first line",
+        "in10",
+        "2 seconds",
+        "5.0 0",
+        "7.0 100",
+        "in1 trailing words",
+        "3 hours",
+        "1.0 0",
+        "2.0 60",
+        "0.5 120",
+        "lvl\r",
+        "1 seconds",
+        "0.3 10",
+        "",
+    ]) + "\n"))
+    pts = c.points()
+    assert pts[0].name == "in1" and list(pts[0].times) == [0, 60, 120]
+    assert list(pts[0].values) == [1.0, 2.0, 0.5]               # value first, then time
+    assert list(pts[1].times) == [0, 100] and list(pts[1].values) == [5.0, 7.0]
+    assert pts[2].times is None                                  # second "in1": no series
+    assert list(pts[3].values) == [0.3]                          # CR after the name matches
+
+
+def test_rain_file(tmp_path):
+    c = formats.Case()
+    c.read_rain(_w(tmp_path, "r.txt", "rate time\n1e-5 0\n# c\n\n2e-5 600\nbad line\n"))
+    en, t, r = c.rain()
+    assert en and list(t) == [0, 600] and list(r) == [1e-5, 2e-5]
+    c2 = formats.Case()
+    c2.read_rain(_w(tmp_path, "r2.txt", "only a header\n"))
+    assert c2.rain()[0] is False
+
+
+def test_interp_matches_reference_rules():
+    from adaptflood_amd.kernels import interpolate_bc_value as f
+    t = np.array([0.0, 10.0, 10.0, 30.0])
+    v = np.array([1.0, 2.0, 5.0, 9.0])
+    assert f(t, v, 0.0, -1) == 1.0 and f(t, v, 0.0, 5) == 1.5 and f(t, v, 0.0, 10) == 5.0
+    assert f(t, v, 0.0, 20) == 7.0 and f(t, v, 0.0, 99) == 9.0 and f(None, None, 3.0, 5) == 3.0
+
+
+def test_write_asc_format(tmp_path):
+    p = str(tmp_path / "o.asc")
+    formats.write_asc(p, np.array([[0.0, 1.23456], [-0.5, 2.0]]), 100.0, 200.5, 2.0)
+    assert open(p).read() == ("ncols         2\nnrows         2\nxllcorner     100.0\n"
+                              "yllcorner     200.5\ncellsize      2.0\nNODATA_value  -9999\n"
+                              "0.0000 1.2346 \n-0.5000 2.0000 \n")
+
+
+def test_round_trip_of_generated_case(tmp_path):
+    """A generated scenario written in the reference formats and read back by
+    the input sequence of main.cpp gives the same scenario."""
+    scn = generators.small_case(23, 17, seed=3, sim_time=250.0)
+    for k, p in enumerate(scn.points):
+        p.name = f"pt{k}"
+    par = formats.write_case(str(tmp_path / "case"), scn, xll=500.0, yll=-300.0)
+    back = formats.load_case(par).scenario()
+    assert np.array_equal(back.dem, scn.dem)
+    assert back.dx == scn.dx and back.sim_time == 250.0 and back.cfl == scn.cfl
+    assert (back.west_free, back.east_free, back.north_free, back.south_free) == \
+           (scn.west_free, scn.east_free, scn.north_free, scn.south_free)
+    assert len(back.points) == len(scn.points)
+    for a, b in zip(back.points, scn.points):
+        assert (a.x, a.y, a.type) == (b.x, b.y, b.type)
+        if b.type in (AF_BC_QFIX, AF_BC_HFIX):
+            assert a.value == b.value
+        if b.times is not None:
+            assert np.array_equal(a.times, b.times) and np.array_equal(a.values, b.values)
+    assert np.array_equal(back.rain_times, scn.rain_times)
+    assert np.array_equal(back.rain_rates, scn.rain_rates)
+    assert back.max_hflow == 10.0 and back.g == 9.81 and back.theta == 1.0
+
+
+def test_missing_inputs_fail_cleanly(tmp_path):
+    c = formats.Case()
+    with pytest.raises(AFError):
+        c.read_dem(str(tmp_path / "nope.asc"))
+    with pytest.raises(AFError):
+        c.read_bci(str(tmp_path / "nope.bci"))   # DEM header needed first
+    par = _w(tmp_path, "p.par", f"DEMfile {tmp_path}/nope.asc\n")
+    with pytest.raises(AFError):
+        formats.load_case(par)
