@@ -108,6 +108,14 @@ SIGNATURES = [
     ("af_batch_read_reports", _I, [_V, _I, _dp, _I32, C.POINTER(_I32), _V]),
     ("af_batch_volume", _I, [_V, _I, _dp, _dp, _V]),
     ("af_batch_launch_step", _I, [_V, _V]),
+    ("af_pool_arena_bytes", _I, [_I, _I, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("af_pool_scenario_bytes", _I, [C.POINTER(ScenarioDesc), _I, C.POINTER(Tuning),
+                                    C.POINTER(C.c_size_t), _ip]),
+    ("af_pool_create", _I, [_I, C.POINTER(Tuning), _I, _I, _V, C.c_size_t, C.POINTER(_V)]),
+    ("af_pool_admit", _I, [_V, C.POINTER(ScenarioDesc), _V, _ip]),
+    ("af_pool_release", _I, [_V, _I32, _V]),
+    ("af_pool_poll", _I, [_V, _ip, _I32, _ip, _ip, _V]),
+    ("af_pool_stats", _I, [_V, _ip, _ip, C.POINTER(C.c_size_t), C.POINTER(_I64)]),
     ("af_par_defaults", None, [C.POINTER(Par)]),
     ("af_read_par", _I, [C.c_char_p, C.POINTER(Par)]),
     ("af_case_new", _V, []),

@@ -51,6 +51,10 @@ struct HostScn {
     std::vector<double> pt_value;
     int n_series = 0, n_rain = 0;
     ScnDev dev{};
+    // pool bookkeeping
+    bool active = false;
+    size_t base = 0, bytes = 0;  // heap extent of this scenario in the arena
+    int base_parity = 0;         // state buffer holding the initial state
 };
 
 }  // namespace
@@ -63,8 +67,11 @@ struct af_domain {
     int report_cap = 64;
     char* arena = nullptr;
     size_t arena_bytes = 0;
-    std::vector<af::HostScn> scn;
-    size_t o_scns = 0, o_tiles = 0, o_volbuf = 0;
+    // pool: a fixed table region, then a heap of scenario extents
+    int max_scn = 0, max_tiles = 0;
+    std::vector<af::HostScn> scn;  // max_scn slots
+    std::vector<std::pair<size_t, size_t>> free_list;  // (offset, size), sorted
+    size_t o_scns = 0, o_tiles = 0, o_volbuf = 0, heap_lo = 0;
     af::ScnDev* d_scns = nullptr;
     af::TileDesc* d_tiles = nullptr;
     double* d_volbuf = nullptr;
@@ -116,6 +123,18 @@ int validate(const af_scenario_desc* d) {
                    "rain enabled without a series");
     }
     return AF_OK;
+}
+
+// Adds the heap base of the scenario's extent to every offset.
+void rebase(HostScn& h, size_t b) {
+    size_t* fields[] = {&h.o_z, &h.o_H[0], &h.o_H[1], &h.o_Qx[0], &h.o_Qx[1], &h.o_Qy[0],
+                        &h.o_Qy[1], &h.o_src_begin, &h.o_src_j, &h.o_src_i, &h.o_src_pt,
+                        &h.o_ovr_begin, &h.o_ovr_j, &h.o_ovr_i, &h.o_ovr_pt, &h.o_pt_type,
+                        &h.o_pt_value, &h.o_pt_off, &h.o_pt_len, &h.o_series_t, &h.o_series_v,
+                        &h.o_rain_t, &h.o_rain_r, &h.o_ctl, &h.o_part_vol, &h.o_part_comp,
+                        &h.o_part_area, &h.o_dt_log, &h.o_reports};
+    for (size_t* f : fields) *f += b;
+    h.base = b;
 }
 
 void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
@@ -238,21 +257,58 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
     return AF_OK;
 }
 
-size_t plan_all(af_domain* D, const af_scenario_desc* descs, int n) {
+// Table region at the start of the arena: scenario table, tile table and
+// the volume-reduction scratch.
+size_t plan_tables(af_domain* D, int max_scn, int max_tiles) {
     Layout L;
-    D->scn.assign(n, HostScn{});
-    int tiles = 0;
-    for (int s = 0; s < n; s++) {
-        plan_geometry(D->scn[s], &descs[s], D);
-        D->scn[s].tile_base = tiles;
-        tiles += D->scn[s].ntiles;
-    }
-    D->ntiles = tiles;
-    D->o_scns = L.take(sizeof(ScnDev) * (size_t)n);
-    D->o_tiles = L.take(sizeof(TileDesc) * (size_t)tiles);
+    D->o_scns = L.take(sizeof(ScnDev) * (size_t)max_scn);
+    D->o_tiles = L.take(sizeof(TileDesc) * (size_t)max_tiles);
     D->o_volbuf = L.take(sizeof(double) * 3 * kVolBlocks);
-    for (int s = 0; s < n; s++) plan_layout(D->scn[s], &descs[s], D, L);
     return L.top;
+}
+
+// Device bytes of one scenario (its heap extent); fills h's geometry and
+// extent-relative offsets.
+size_t scenario_bytes(const af_domain* D, const af_scenario_desc* d, HostScn& h) {
+    plan_geometry(h, d, D);
+    Layout L;
+    plan_layout(h, d, D, L);
+    return align_up(L.top, 256);
+}
+
+bool heap_alloc(af_domain* D, size_t bytes, size_t* off) {
+    for (size_t i = 0; i < D->free_list.size(); i++) {
+        auto& f = D->free_list[i];
+        if (f.second >= bytes) {
+            *off = f.first;
+            f.first += bytes;
+            f.second -= bytes;
+            if (f.second == 0) D->free_list.erase(D->free_list.begin() + i);
+            return true;
+        }
+    }
+    return false;
+}
+
+void heap_free(af_domain* D, size_t off, size_t bytes) {
+    auto& fl = D->free_list;
+    auto it = std::lower_bound(fl.begin(), fl.end(), std::make_pair(off, (size_t)0));
+    it = fl.insert(it, {off, bytes});
+    size_t i = it - fl.begin();
+    if (i + 1 < fl.size() && fl[i].first + fl[i].second == fl[i + 1].first) {
+        fl[i].second += fl[i + 1].second;
+        fl.erase(fl.begin() + i + 1);
+    }
+    if (i > 0 && fl[i - 1].first + fl[i - 1].second == fl[i].first) {
+        fl[i - 1].second += fl[i].second;
+        fl.erase(fl.begin() + i);
+    }
+}
+
+size_t heap_free_bytes(const af_domain* D) {
+    size_t n = 0;
+    for (auto& f : D->free_list) n += f.second;
+    return n;
 }
 
 template <typename T>
@@ -304,7 +360,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     v.P = h.P;
     v.nstrips = h.nstrips;
     v.tile_rows = D->R;
-    v.tile_base = h.tile_base;
+    v.tile_base = 0;
     v.ntiles = h.ntiles;
     v.z = A + h.o_z;
     for (int b = 0; b < 2; b++) {
@@ -436,6 +492,7 @@ int launch_step_v(af_domain* D, int kmod6, hipStream_t st) {
 // PF = 2: two-row load and store bursts) and the occupancy (waves per SIMD)
 // the register allocation targets.
 int launch_step(af_domain* D, int kmod6, hipStream_t st) {
+    if (D->ntiles == 0) return AF_OK;
     if (D->precision == AF_FP64) {
         switch (D->variant) {
             case 1: return launch_step_v<double, 2, true, 2, 2>(D, kmod6, st);
@@ -512,18 +569,112 @@ int current_parity(af_domain* D, int s, long long* n_done, hipStream_t st) {
     AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     AF_HIP(hipStreamSynchronize(st));
     if (n_done) *n_done = c.n_done;
-    return (int)(c.n_done & 1);
+    return (int)((D->scn[s].base_parity + c.n_done) & 1);
 }
 
-__global__ void k_finalize_reports(const ScnDev* __restrict__ scns, int n) {
-    const int lane = threadIdx.x & 63;
-    for (int s = 0; s < n; s++) Stepper<double, 2, true>::finalize_report(scns[s], lane);
+__global__ void k_finalize_reports(const ScnDev* __restrict__ scn) {
+    Stepper<double, 2, true>::finalize_report(*scn, threadIdx.x & 63);
 }
 
-int finalize_reports(af_domain* D, hipStream_t st) {
-    hipLaunchKernelGGL(k_finalize_reports, dim3(1), dim3(64), 0, st, D->d_scns,
-                       (int)D->scn.size());
+// Completes a mass report whose partial sums the last executed step left.
+int finalize_reports(af_domain* D, int s, hipStream_t st) {
+    hipLaunchKernelGGL(k_finalize_reports, dim3(1), dim3(64), 0, st, D->d_scns + s);
     AF_HIP(hipGetLastError());
+    return AF_OK;
+}
+
+// Rebuilds and uploads the tile table from the active scenarios; a captured
+// graph holds the tile count, so it is dropped.
+int rebuild_tiles(af_domain* D, hipStream_t st) {
+    std::vector<TileDesc> tiles;
+    for (int s = 0; s < D->max_scn; s++) {
+        const HostScn& h = D->scn[s];
+        if (!h.active) continue;
+        for (int seg = 0; seg < h.nsegs; seg++)
+            for (int strip = 0; strip < h.nstrips; strip++)
+                tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
+    }
+    AF_REQUIRE((int)tiles.size() <= D->max_tiles, AF_E_CAPACITY, "tile table full (%zu > %d)",
+               tiles.size(), D->max_tiles);
+    if (!tiles.empty())
+        AF_HIP(hipMemcpyAsync(D->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
+                              hipMemcpyHostToDevice, st));
+    AF_HIP(hipStreamSynchronize(st));
+    D->ntiles = (int)tiles.size();
+    if (D->gexec) {
+        (void)hipGraphExecDestroy(D->gexec);
+        D->gexec = nullptr;
+    }
+    if (D->graph) {
+        (void)hipGraphDestroy(D->graph);
+        D->graph = nullptr;
+    }
+    return AF_OK;
+}
+
+int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn, int max_tiles,
+              void* arena, size_t arena_bytes) {
+    int rc = configure(D, precision, tuning);
+    if (rc) return rc;
+    AF_REQUIRE(max_scn >= 1 && max_tiles >= 1, AF_E_INVALID, "pool needs >= 1 scenario and tile");
+    AF_REQUIRE(arena && ((uintptr_t)arena & 255) == 0, AF_E_INVALID,
+               "arena must be 256-byte aligned device memory");
+    const size_t tables = plan_tables(D, max_scn, max_tiles);
+    AF_REQUIRE(tables <= arena_bytes, AF_E_CAPACITY, "arena too small: tables need %zu bytes",
+               tables);
+    D->max_scn = max_scn;
+    D->max_tiles = max_tiles;
+    D->scn.assign(max_scn, HostScn{});
+    D->arena = reinterpret_cast<char*>(arena);
+    D->arena_bytes = arena_bytes;
+    D->heap_lo = tables;
+    D->free_list.clear();
+    if (arena_bytes > tables) D->free_list.push_back({tables, arena_bytes - tables});
+    D->d_scns = reinterpret_cast<ScnDev*>(D->arena + D->o_scns);
+    D->d_tiles = reinterpret_cast<TileDesc*>(D->arena + D->o_tiles);
+    D->d_volbuf = reinterpret_cast<double*>(D->arena + D->o_volbuf);
+    return AF_OK;
+}
+
+// Places a scenario in a free slot and heap extent and uploads it; its state
+// enters at the pool's current step index k.
+int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool rebuild, int* slot) {
+    int rc = validate(d);
+    if (rc) return rc;
+    int s = -1;
+    for (int i = 0; i < D->max_scn; i++)
+        if (!D->scn[i].active) { s = i; break; }
+    AF_REQUIRE(s >= 0, AF_E_CAPACITY, "no free scenario slot (%d in use)", D->max_scn);
+    HostScn h;
+    const size_t bytes = scenario_bytes(D, d, h);
+    int used = 0;
+    for (const HostScn& o : D->scn) used += o.active ? o.ntiles : 0;
+    AF_REQUIRE(used + h.ntiles <= D->max_tiles, AF_E_CAPACITY,
+               "tile table full: %d in use, %d more needed, %d max", used, h.ntiles,
+               D->max_tiles);
+    size_t off = 0;
+    AF_REQUIRE(heap_alloc(D, bytes, &off), AF_E_CAPACITY,
+               "arena full: %zu bytes needed, %zu free", bytes, heap_free_bytes(D));
+    rebase(h, off);
+    h.bytes = bytes;
+    D->scn[s] = std::move(h);
+    HostScn& hs = D->scn[s];
+    const int parity = (int)(D->k & 1);
+    rc = upload_scenario(D, s, d, st, parity, D->k % 6);
+    if (rc) {
+        heap_free(D, off, bytes);
+        D->scn[s] = HostScn{};
+        return rc;
+    }
+    hs.base_parity = parity;
+    hs.active = true;
+    AF_HIP(hipMemcpyAsync(D->d_scns + s, &hs.dev, sizeof(ScnDev), hipMemcpyHostToDevice, st));
+    AF_HIP(hipStreamSynchronize(st));
+    if (rebuild) {
+        rc = rebuild_tiles(D, st);
+        if (rc) return rc;
+    }
+    *slot = s;
     return AF_OK;
 }
 
@@ -540,11 +691,16 @@ int af_batch_arena_bytes(const af_scenario_desc* descs, int n, int precision,
     af_domain tmp;
     int rc = configure(&tmp, precision, tuning);
     if (rc) return rc;
+    size_t heap = 0;
+    int tiles = 0;
     for (int s = 0; s < n; s++) {
         rc = validate(&descs[s]);
         if (rc) return rc;
+        HostScn h;
+        heap += scenario_bytes(&tmp, &descs[s], h);
+        tiles += h.ntiles;
     }
-    *bytes = plan_all(&tmp, descs, n);
+    *bytes = plan_tables(&tmp, n, tiles) + heap;
     return AF_OK;
 }
 
@@ -556,48 +712,31 @@ int af_domain_arena_bytes(const af_scenario_desc* desc, int precision, const af_
 int af_batch_create(const af_scenario_desc* descs, int n, int precision, const af_tuning* tuning,
                     void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out) {
     AF_REQUIRE(descs && n >= 1 && arena && out, AF_E_INVALID, "create: bad arguments");
-    AF_REQUIRE(((uintptr_t)arena & 255) == 0, AF_E_INVALID, "arena must be 256-byte aligned");
+    size_t need = 0;
+    int rc = af_batch_arena_bytes(descs, n, precision, tuning, &need);
+    if (rc) return rc;
+    AF_REQUIRE(need <= arena_bytes, AF_E_CAPACITY, "arena too small: need %zu bytes, got %zu",
+               need, arena_bytes);
+    af_domain tmp;
+    (void)configure(&tmp, precision, tuning);
+    int tiles = 0;
+    for (int s = 0; s < n; s++) {
+        HostScn h;
+        (void)scenario_bytes(&tmp, &descs[s], h);
+        tiles += h.ntiles;
+    }
     af_domain* D = new af_domain();
-    int rc = configure(D, precision, tuning);
-    for (int s = 0; s < n && rc == AF_OK; s++) rc = validate(&descs[s]);
-    if (rc) { delete D; return rc; }
-    size_t need = plan_all(D, descs, n);
-    if (need > arena_bytes) {
-        delete D;
-        set_error("arena too small: need %zu bytes, got %zu", need, arena_bytes);
-        return AF_E_CAPACITY;
-    }
-    D->arena = reinterpret_cast<char*>(arena);
-    D->arena_bytes = arena_bytes;
-    D->d_scns = reinterpret_cast<ScnDev*>(D->arena + D->o_scns);
-    D->d_tiles = reinterpret_cast<TileDesc*>(D->arena + D->o_tiles);
-    D->d_volbuf = reinterpret_cast<double*>(D->arena + D->o_volbuf);
     hipStream_t st = to_stream(stream);
-    std::vector<ScnDev> devs(n);
-    std::vector<TileDesc> tiles;
-    tiles.reserve(D->ntiles);
+    rc = pool_init(D, precision, tuning, n, tiles, arena, arena_bytes);
     for (int s = 0; s < n && rc == AF_OK; s++) {
-        rc = upload_scenario(D, s, &descs[s], st, 0, 0);
-        if (rc) break;
-        devs[s] = D->scn[s].dev;
-        const HostScn& h = D->scn[s];
-        for (int seg = 0; seg < h.nsegs; seg++)
-            for (int strip = 0; strip < h.nstrips; strip++)
-                tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
+        int slot = -1;
+        rc = pool_admit(D, &descs[s], st, false, &slot);
     }
-    if (rc == AF_OK) {
-        hipError_t e = hipMemcpyAsync(D->d_scns, devs.data(), sizeof(ScnDev) * n,
-                                      hipMemcpyHostToDevice, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(D->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
-                               hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) {
-            set_error("upload of tables: %s", hipGetErrorString(e));
-            rc = AF_E_HIP;
-        }
+    if (rc == AF_OK) rc = rebuild_tiles(D, st);
+    if (rc) {
+        delete D;
+        return rc;
     }
-    if (rc) { delete D; return rc; }
     *out = D;
     return AF_OK;
 }
@@ -605,6 +744,94 @@ int af_batch_create(const af_scenario_desc* descs, int n, int precision, const a
 int af_domain_create(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
                      void* arena, size_t arena_bytes, af_stream_t stream, af_domain** out) {
     return af_batch_create(desc, 1, precision, tuning, arena, arena_bytes, stream, out);
+}
+
+int af_pool_arena_bytes(int max_scenarios, int max_tiles, size_t heap_bytes, size_t* bytes) {
+    AF_REQUIRE(bytes && max_scenarios >= 1 && max_tiles >= 1, AF_E_INVALID,
+               "pool_arena_bytes: bad arguments");
+    af_domain tmp;
+    *bytes = plan_tables(&tmp, max_scenarios, max_tiles) + align_up(heap_bytes, 256);
+    return AF_OK;
+}
+
+int af_pool_scenario_bytes(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                           size_t* bytes, int32_t* tiles) {
+    AF_REQUIRE(desc && bytes, AF_E_INVALID, "scenario_bytes: bad arguments");
+    af_domain tmp;
+    int rc = configure(&tmp, precision, tuning);
+    if (rc) return rc;
+    rc = validate(desc);
+    if (rc) return rc;
+    HostScn h;
+    *bytes = scenario_bytes(&tmp, desc, h);
+    if (tiles) *tiles = h.ntiles;
+    return AF_OK;
+}
+
+int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, int max_tiles,
+                   void* arena, size_t arena_bytes, af_domain** out) {
+    AF_REQUIRE(out, AF_E_INVALID, "pool_create: null out");
+    af_domain* D = new af_domain();
+    int rc = pool_init(D, precision, tuning, max_scenarios, max_tiles, arena, arena_bytes);
+    if (rc) {
+        delete D;
+        return rc;
+    }
+    *out = D;
+    return AF_OK;
+}
+
+int af_pool_admit(af_domain* D, const af_scenario_desc* desc, af_stream_t stream, int32_t* slot) {
+    AF_REQUIRE(D && desc && slot, AF_E_INVALID, "admit: bad arguments");
+    int s = -1;
+    int rc = pool_admit(D, desc, to_stream(stream), true, &s);
+    if (rc) return rc;
+    *slot = s;
+    return AF_OK;
+}
+
+int af_pool_release(af_domain* D, int32_t slot, af_stream_t stream) {
+    AF_REQUIRE(D && slot >= 0 && slot < D->max_scn && D->scn[slot].active, AF_E_INVALID,
+               "release: slot %d is not in use", slot);
+    HostScn& h = D->scn[slot];
+    heap_free(D, h.base, h.bytes);
+    h = HostScn{};
+    return rebuild_tiles(D, to_stream(stream));
+}
+
+int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finished,
+                 int32_t* n_active, af_stream_t stream) {
+    AF_REQUIRE(D && n_finished, AF_E_INVALID, "poll: bad arguments");
+    hipStream_t st = to_stream(stream);
+    const int par = (int)(D->k & 1);
+    int nf = 0, na = 0;
+    for (int s = 0; s < D->max_scn; s++) {
+        if (!D->scn[s].active) continue;
+        na++;
+        double t = 0.0;
+        AF_HIP(hipMemcpyAsync(&t, &D->scn[s].dev.ctl->t[par], sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        AF_HIP(hipStreamSynchronize(st));
+        if (!(t < D->scn[s].dev.sim_time)) {
+            if (finished && nf < cap) finished[nf] = s;
+            nf++;
+        }
+    }
+    *n_finished = nf;
+    if (n_active) *n_active = na;
+    return AF_OK;
+}
+
+int af_pool_stats(af_domain* D, int32_t* active, int32_t* tiles, size_t* heap_free,
+                  int64_t* step_index) {
+    AF_REQUIRE(D, AF_E_INVALID, "stats: null pool");
+    int na = 0;
+    for (const HostScn& h : D->scn) na += h.active ? 1 : 0;
+    if (active) *active = na;
+    if (tiles) *tiles = D->ntiles;
+    if (heap_free) *heap_free = heap_free_bytes(D);
+    if (step_index) *step_index = D->k;
+    return AF_OK;
 }
 
 int af_domain_destroy(af_domain* D) {
@@ -641,15 +868,17 @@ int af_domain_advance(af_domain* D, int64_t max_steps, int use_graph, af_stream_
 int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
     AF_REQUIRE(D && all_done, AF_E_INVALID, "finished: bad arguments");
     hipStream_t st = to_stream(stream);
-    const int n = (int)D->scn.size();
+    const int n = D->max_scn;
     std::vector<Ctl> c(n);
     for (int s = 0; s < n; s++)
-        AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        if (D->scn[s].active)
+            AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
+                                  st));
     AF_HIP(hipStreamSynchronize(st));
     int done = 1;
     for (int s = 0; s < n; s++) {
         const int par = (int)(D->k & 1);
-        if (c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
+        if (D->scn[s].active && c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
     }
     *all_done = done;
     return AF_OK;
@@ -658,18 +887,21 @@ int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
 int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t stream) {
     AF_REQUIRE(D && check_every > 0, AF_E_INVALID, "run: bad arguments");
     hipStream_t st = to_stream(stream);
-    const int n = (int)D->scn.size();
+    const int n = D->max_scn;
     std::vector<Ctl> c(n);
     long long prev_done = -1;
     for (;;) {
         for (int s = 0; s < n; s++)
-            AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+            if (D->scn[s].active)
+                AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl),
+                                      hipMemcpyDeviceToHost, st));
         AF_HIP(hipStreamSynchronize(st));
         const int par = (int)(D->k & 1);
         int done = 1;
         long long steps = 0;
         double tsum = 0.0;
         for (int s = 0; s < n; s++) {
+            if (!D->scn[s].active) continue;
             if (c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
             steps += c[s].n_done;
             tsum += c[s].t[par];
@@ -689,9 +921,10 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
 }
 
 int af_batch_info(af_domain* D, int s, af_run_info* info, af_stream_t stream) {
-    AF_REQUIRE(D && info && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "info: bad arguments");
+    AF_REQUIRE(D && info && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
+               "info: bad arguments");
     hipStream_t st = to_stream(stream);
-    int rc = finalize_reports(D, st);
+    int rc = finalize_reports(D, s, st);
     if (rc) return rc;
     Ctl c;
     AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
@@ -711,7 +944,8 @@ int af_domain_info(af_domain* D, af_run_info* info, af_stream_t stream) {
 }
 
 int af_batch_read(af_domain* D, int s, int which, double* out, af_stream_t stream) {
-    AF_REQUIRE(D && out && s >= 0 && s < (int)D->scn.size() && which >= 0 && which <= 4,
+    AF_REQUIRE(D && out && s >= 0 && s < D->max_scn && D->scn[s].active && which >= 0 &&
+                   which <= 4,
                AF_E_INVALID, "read: bad arguments");
     hipStream_t st = to_stream(stream);
     HostScn& h = D->scn[s];
@@ -764,7 +998,8 @@ int af_domain_read(af_domain* D, int which, double* out, af_stream_t stream) {
 
 int af_batch_read_dt_log(af_domain* D, int s, double* out, int64_t cap, int64_t* n,
                          af_stream_t stream) {
-    AF_REQUIRE(D && n && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "dt_log: bad arguments");
+    AF_REQUIRE(D && n && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
+               "dt_log: bad arguments");
     hipStream_t st = to_stream(stream);
     long long done = 0;
     if (current_parity(D, s, &done, st) < 0) return AF_E_HIP;
@@ -783,10 +1018,10 @@ int af_domain_read_dt_log(af_domain* D, double* out, int64_t cap, int64_t* n,
 
 int af_batch_read_reports(af_domain* D, int s, double* out, int32_t cap, int32_t* n,
                           af_stream_t stream) {
-    AF_REQUIRE(D && n && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID,
+    AF_REQUIRE(D && n && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "reports: bad arguments");
     hipStream_t st = to_stream(stream);
-    int rc = finalize_reports(D, st);
+    int rc = finalize_reports(D, s, st);
     if (rc) return rc;
     Ctl c;
     AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
@@ -805,7 +1040,8 @@ int af_domain_read_reports(af_domain* D, double* out, int32_t cap, int32_t* n,
 }
 
 int af_batch_volume(af_domain* D, int s, double* vol, double* area, af_stream_t stream) {
-    AF_REQUIRE(D && s >= 0 && s < (int)D->scn.size(), AF_E_INVALID, "volume: bad arguments");
+    AF_REQUIRE(D && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
+               "volume: bad arguments");
     hipStream_t st = to_stream(stream);
     HostScn& h = D->scn[s];
     const int par = current_parity(D, s, nullptr, st);
@@ -850,7 +1086,9 @@ int af_domain_state_ptr(af_domain* D, int which, void** ptr, int64_t* pitch_elem
 
 int af_batch_size(af_domain* D, int* n_scenarios, int* n_tiles) {
     AF_REQUIRE(D, AF_E_INVALID, "batch_size: null");
-    if (n_scenarios) *n_scenarios = (int)D->scn.size();
+    int na = 0;
+    for (const HostScn& h : D->scn) na += h.active ? 1 : 0;
+    if (n_scenarios) *n_scenarios = na;
     if (n_tiles) *n_tiles = D->ntiles;
     return AF_OK;
 }

@@ -1,0 +1,191 @@
+"""Continuous batching of flood scenarios on one GPU.
+
+A Pool is one device arena (a torch uint8 tensor) managed by the native pool
+(af_pool_*): scenarios are admitted while the HBM budget, the scenario slots
+and the tile table have room, every admitted scenario advances by the same
+fused step launches (each with its own adaptive dt and clock), and finished
+scenarios are read back and released between launches so waiting requests
+can take their place.
+"""
+import collections
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+from .domain import PRECISIONS, _stream, _torch, make_tuning
+
+AF_E_CAPACITY = -4
+
+
+class Pool:
+    def __init__(self, budget_bytes, precision="fp32", max_scenarios=1024, max_tiles=1 << 20,
+                 tuning=None, device=None, stream=None):
+        torch = _torch()
+        self._torch = torch
+        self.precision = PRECISIONS[precision]
+        self.tuning = tuning if tuning is not None else make_tuning()
+        n = C.c_size_t()
+        call("af_pool_arena_bytes", int(max_scenarios), int(max_tiles), C.c_size_t(int(budget_bytes)),
+             C.byref(n))
+        self.arena_bytes = int(n.value)
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.arena = torch.empty(self.arena_bytes + 256, dtype=torch.uint8, device=dev)
+        aligned = (self.arena.data_ptr() + 255) // 256 * 256
+        self._stream = stream
+        h = C.c_void_p()
+        call("af_pool_create", self.precision, C.byref(self.tuning), int(max_scenarios),
+             int(max_tiles), C.c_void_p(aligned), C.c_size_t(self.arena_bytes), C.byref(h))
+        self.handle = h
+        self.packed = {}
+
+    def _s(self):
+        return _stream(self._torch, self._stream)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.lib().af_domain_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self, scn):
+        """(device bytes, tile-table entries) the scenario needs."""
+        pk = scn.pack()
+        b = C.c_size_t()
+        t = C.c_int32()
+        call("af_pool_scenario_bytes", C.byref(pk.desc), self.precision, C.byref(self.tuning),
+             C.byref(b), C.byref(t))
+        return int(b.value), int(t.value)
+
+    def admit(self, scn):
+        """Slot index, or None when the pool has no room right now."""
+        pk = scn.pack()
+        slot = C.c_int32()
+        rc = _lib.lib().af_pool_admit(self.handle, C.byref(pk.desc), self._s(), C.byref(slot))
+        if rc == AF_E_CAPACITY:
+            return None
+        _lib.check("af_pool_admit", rc)
+        self.packed[slot.value] = (scn, pk)
+        return slot.value
+
+    def release(self, slot):
+        call("af_pool_release", self.handle, int(slot), self._s())
+        self.packed.pop(slot, None)
+
+    def advance(self, steps, graph=True):
+        call("af_domain_advance", self.handle, int(steps), 1 if graph else 0, self._s())
+
+    def poll(self):
+        """(finished slots, number of active slots)."""
+        cap = max(1, len(self.packed))
+        buf = (C.c_int32 * cap)()
+        nf, na = C.c_int32(), C.c_int32()
+        call("af_pool_poll", self.handle, buf, cap, C.byref(nf), C.byref(na), self._s())
+        return [buf[i] for i in range(min(nf.value, cap))], na.value
+
+    def stats(self):
+        a, t = C.c_int32(), C.c_int32()
+        hf = C.c_size_t()
+        k = C.c_int64()
+        call("af_pool_stats", self.handle, C.byref(a), C.byref(t), C.byref(hf), C.byref(k))
+        return {"active": a.value, "tiles": t.value, "heap_free": int(hf.value),
+                "step_index": k.value}
+
+    def info(self, slot):
+        r = _lib.RunInfo()
+        call("af_batch_info", self.handle, int(slot), C.byref(r), self._s())
+        return {"t": r.t, "steps": r.steps_done, "last_dt": r.last_dt, "hmax": r.hmax}
+
+    def read(self, slot, what="H"):
+        which = {"H": 0, "Qx": 1, "Qy": 2, "Qxold": 3, "Qyold": 4}[what]
+        scn = self.packed[slot][0]
+        if which == 0:
+            out = np.empty((scn.ny, scn.nx))
+        else:
+            out = np.empty((scn.ny + 1) * (scn.nx + 1))
+        call("af_batch_read", self.handle, int(slot), which, _lib.dptr(out), self._s())
+        return out
+
+
+class Request:
+    __slots__ = ("rid", "scenario", "arrival", "admitted", "done", "steps", "cells", "result")
+
+    def __init__(self, rid, scenario, arrival):
+        self.rid, self.scenario, self.arrival = rid, scenario, arrival
+        self.admitted = self.done = None
+        self.steps = 0
+        self.cells = scenario.cells
+        self.result = None
+
+
+def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, graph=True):
+    """Runs requests (a list of Request, sorted by arrival) through the pool.
+
+    clock="wall": arrivals are seconds on the wall clock from the start of the
+    call (a request is admitted once it has arrived); clock="steps": arrivals
+    are pool step indices (a deterministic schedule for tests).  Returns the
+    requests with admission/completion times filled in and a summary."""
+    torch = pool._torch
+    pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+    waiting = collections.deque()
+    active = {}
+    done = []
+    t0 = time.perf_counter()
+    step = 0
+    cell_updates = 0
+
+    def now():
+        return time.perf_counter() - t0 if clock == "wall" else float(step)
+
+    while pending or waiting or active:
+        n = now()
+        while pending and pending[0].arrival <= n:
+            waiting.append(pending.popleft())
+        while waiting:
+            r = waiting[0]
+            slot = pool.admit(r.scenario)
+            if slot is None:
+                break
+            waiting.popleft()
+            r.admitted = now()
+            active[slot] = r
+        if not active:
+            if clock == "wall":
+                if pending:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now())))
+            else:
+                step = int(max(step + 1, pending[0].arrival)) if pending else step + 1
+            continue
+        pool.advance(chunk_steps, graph=graph)
+        step += chunk_steps
+        finished, _ = pool.poll()
+        for slot in finished:
+            r = active.pop(slot)
+            inf = pool.info(slot)
+            r.steps = inf["steps"]
+            cell_updates += r.steps * r.cells
+            if keep_results:
+                r.result = pool.read(slot, "H")
+            r.done = now()
+            pool.release(slot)
+            done.append(r)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([r.done - r.arrival for r in done]) if done else np.zeros(1)
+    summary = {
+        "requests": len(done),
+        "wall_s": wall,
+        "cell_updates": cell_updates,
+        "gcups": cell_updates / wall / 1e9 if clock == "wall" else None,
+        "p50_latency": float(np.percentile(lat, 50)),
+        "p99_latency": float(np.percentile(lat, 99)),
+        "clock": clock,
+    }
+    return done, summary

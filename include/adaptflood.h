@@ -235,6 +235,27 @@ int af_batch_volume(af_domain* d, int scenario, double* vol, double* area, af_st
 /* Enqueues exactly one fused step without a graph (for timing one launch). */
 int af_batch_launch_step(af_domain* d, af_stream_t stream);
 
+/* Pools: one fixed arena (a table region, then a heap of scenario extents)
+ * into which scenarios are admitted and from which they are released while
+ * the others keep stepping: continuous batching.  A pool is an af_domain;
+ * every af_domain_* / af_batch_* call applies to it, with slot indices as
+ * scenario indices.  Admission and release take effect between launches. */
+int af_pool_arena_bytes(int max_scenarios, int max_tiles, size_t heap_bytes, size_t* bytes);
+/* Heap bytes and tile-table entries one scenario needs (its HBM footprint). */
+int af_pool_scenario_bytes(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
+                           size_t* bytes, int32_t* tiles);
+int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, int max_tiles,
+                   void* arena, size_t arena_bytes, af_domain** out);
+/* AF_E_CAPACITY when no slot, tile-table room or heap extent is free. */
+int af_pool_admit(af_domain* pool, const af_scenario_desc* desc, af_stream_t stream,
+                  int32_t* slot);
+int af_pool_release(af_domain* pool, int32_t slot, af_stream_t stream);
+/* Slots whose clock reached their sim_time (up to cap), and the active count. */
+int af_pool_poll(af_domain* pool, int32_t* finished, int32_t cap, int32_t* n_finished,
+                 int32_t* n_active, af_stream_t stream);
+int af_pool_stats(af_domain* pool, int32_t* active, int32_t* tiles, size_t* heap_free,
+                  int64_t* step_index);
+
 /* ---------------------------------------------------------------------- */
 /* Input formats (host): .par, .dem.ascii, .bci, .bdy, rain                */
 /* ---------------------------------------------------------------------- */

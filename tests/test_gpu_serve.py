@@ -1,0 +1,52 @@
+"""Continuous batching (GPU): each served result equals the scenario run alone."""
+import numpy as np
+import pytest
+
+from adaptflood_amd import generators
+from adaptflood_amd.domain import Domain, make_tuning
+from adaptflood_amd.serve import Pool, Request, serve
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenarios():
+    sizes = [(60, 41), (130, 70), (17, 90), (257, 33), (45, 45), (90, 120)]
+    return [generators.small_case(nx, ny, seed=70 + k, sim_time=60.0 + 25 * k)
+            for k, (nx, ny) in enumerate(sizes)]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_served_equals_alone(gpu, prec):
+    scns = _scenarios()
+    tun = make_tuning(tile_rows=16)
+    # arrivals staggered over pool steps; the budget admits only some at once
+    reqs = [Request(k, s, arrival=7 * k) for k, s in enumerate(scns)]
+    pool = Pool(budget_bytes=3 << 20, precision=prec, max_scenarios=4, max_tiles=512, tuning=tun)
+    done, summary = serve(reqs, pool, chunk_steps=6, clock="steps", keep_results=True)
+    assert summary["requests"] == len(scns)
+    for r in done:
+        d = Domain(r.scenario, precision=prec, tuning=tun)
+        d.run(check_every=24, graph=True)
+        assert r.steps == d.info()["steps"], r.rid
+        assert np.array_equal(r.result, d.read("H")), r.rid
+
+
+def test_pool_capacity_and_release(gpu):
+    scns = _scenarios()[:3]
+    pool = Pool(budget_bytes=1 << 20, precision="fp32", max_scenarios=2, max_tiles=64,
+                tuning=make_tuning(tile_rows=16))
+    b0 = [pool.footprint(s)[0] for s in scns]
+    assert all(b > 0 for b in b0)
+    s0 = pool.admit(scns[0])
+    s1 = pool.admit(scns[1])
+    assert s0 is not None and s1 is not None
+    assert pool.admit(scns[2]) is None          # two slots only
+    st = pool.stats()
+    assert st["active"] == 2 and st["tiles"] > 0
+    pool.release(s0)
+    assert pool.stats()["active"] == 1
+    s2 = pool.admit(scns[2])
+    assert s2 == s0                              # the freed slot is reused
+    pool.advance(12, graph=False)
+    fin, act = pool.poll()
+    assert act == 2
