@@ -178,7 +178,8 @@ typedef struct af_tuning {
     int32_t graph_steps;     /* steps per captured graph (multiple of 6); 0 = default */
     int32_t dt_log_cap;      /* device dt log entries kept (0 = none) */
     int32_t report_cap;      /* device mass-report records kept */
-    int32_t variant;         /* kernel variant 0..2: target occupancy of the step kernel */
+    int32_t variant;         /* 0 = library default; 1..3 = step-kernel variant: 1 one-row
+                              * loop, 2 and 3 two-row bursts at lower target occupancy */
 } af_tuning;
 
 /* Device bytes the domain needs for `desc` at `precision`. */

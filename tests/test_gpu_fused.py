@@ -27,6 +27,7 @@ def _faces_close(a, b, tol):
 
 @pytest.mark.parametrize("seed,nx,ny,rows", [
     (1, 37, 29, 8), (2, 64, 64, 16), (3, 130, 70, 32), (4, 257, 19, 5), (5, 300, 300, 128),
+    (6, 1100, 40, 16),
 ])
 def test_fp64_matches_oracle_small(gpu, seed, nx, ny, rows):
     scn = _small(seed, nx, ny, sim_time=200.0)
@@ -107,11 +108,13 @@ def test_graph_equals_eager(gpu):
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_tiling_invariance(gpu, prec):
-    """Results do not depend on the tile shape (bitwise)."""
-    scn = _small(31, 333, 141, sim_time=120.0)
+    """Results do not depend on the tile shape or the kernel variant (bitwise);
+    a wide grid (nx = 2100) takes the 1 KiB row pitch."""
+    scn = _small(31, 2100, 141, sim_time=60.0)
     outs = []
-    for rows, wpb in ((4, 1), (32, 4), (128, 4)):
-        d = Domain(scn, precision=prec, tuning=make_tuning(tile_rows=rows, waves_per_block=wpb))
+    for rows, wpb, var in ((4, 1, 1), (32, 4, 0), (128, 4, 2), (61, 4, 3)):
+        d = Domain(scn, precision=prec,
+                   tuning=make_tuning(tile_rows=rows, waves_per_block=wpb, variant=var))
         d.run(check_every=25, graph=False)
         outs.append((d.read("H"), d.read("Qx"), d.read("Qy")))
     for o in outs[1:]:
