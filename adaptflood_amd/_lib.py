@@ -103,6 +103,7 @@ SIGNATURES = [
     ("af_batch_size", _I, [_V, C.POINTER(_I), C.POINTER(_I)]),
     ("af_batch_finished", _I, [_V, C.POINTER(_I), _V]),
     ("af_batch_info", _I, [_V, _I, C.POINTER(RunInfo), _V]),
+    ("af_batch_set_sim_time", _I, [_V, _I, C.c_double, _V]),
     ("af_batch_read", _I, [_V, _I, _I, _dp, _V]),
     ("af_batch_read_dt_log", _I, [_V, _I, _dp, _I64, C.POINTER(_I64), _V]),
     ("af_batch_read_reports", _I, [_V, _I, _dp, _I32, C.POINTER(_I32), _V]),

@@ -1,0 +1,370 @@
+// driver.cpp -- lisflood_amd: the reference program (src/main.cpp) over the
+// MI355X library.
+//
+//   lisflood_amd [--path compat|fp64|fp32] [--quiet] <parfile>
+//
+// --path compat (default): the reference's own call sequence
+//   (main.cpp:227-288) through the drop-in wrappers of adaptflood_compat.hpp,
+//   FP64 on the reference memory layout, one synchronous call per kernel.
+// --path fp64 | fp32: the fused one-launch-per-step domain, replayed from
+//   HIP graphs; it stops at each save time (the first step with
+//   t >= next_save, as main.cpp:327 saves) to write the output.
+//
+// Console lines and output files follow main.cpp: the mass report line
+// "t=%.0fs: Tstep=%.2f, Vol=%.0fm3, Area=%.0fm2" (main.cpp:300), H_%04d.asc
+// at t = 0 and at every save, H_final.asc, Vx_final.asc / Vy_final.asc when
+// the .par file asks for velocity output (main.cpp:380-392).
+#include <hip/hip_runtime_api.h>
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adaptflood.h"
+#include "adaptflood_compat.hpp"
+
+namespace {
+
+struct Fail {
+    std::string msg;
+};
+
+void hip_ok(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Fail{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void af_ok(int rc, const char* what) {
+    if (rc != AF_OK) throw Fail{std::string(what) + ": " + af_last_error()};
+}
+
+template <typename T>
+T* dev_alloc(size_t n) {
+    void* p = nullptr;
+    hip_ok(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc");
+    hip_ok(hipMemset(p, 0, (n ? n : 1) * sizeof(T)), "hipMemset");
+    return static_cast<T*>(p);
+}
+
+struct Grid {
+    int nx = 0, ny = 0;
+    double xll = 0, yll = 0, cell = 0, nodata = 0;
+};
+
+struct Output {
+    std::string dir;
+    Grid g;
+    bool quiet = false;
+    void asc(const char* name, const std::vector<double>& data, const char* note = "") const {
+        const std::string path = dir + "/" + name;
+        af_ok(af_write_asc(path.c_str(), data.data(), g.nx, g.ny, g.xll, g.yll, g.cell),
+              "write_asc");
+        if (!quiet) printf("Saved: %s%s\n", path.c_str(), note);
+    }
+};
+
+void report_line(double t, double dt, double vol, double area) {
+    printf("t=%.0fs: Tstep=%.2f, Vol=%.0fm3, Area=%.0fm2\n", t, dt, vol, area);
+    fflush(stdout);
+}
+
+// Host volume and flooded area of a host copy of H (main.cpp:292-298).
+void mass_of(const std::vector<double>& H, double dA, double* vol, double* area) {
+    double v = 0.0, a = 0.0;
+    for (double h : H) {
+        v += h * dA;
+        if (h > 0.01) a += dA;
+    }
+    *vol = v;
+    *area = a;
+}
+
+// Device velocity from a reference-layout state (kernels.cpp:293-351).
+void velocity_files(const Output& out, const std::vector<double>& H, const std::vector<double>& dem,
+                    const std::vector<double>& Qx, const std::vector<double>& Qy, double dx) {
+    const size_t n = H.size(), nq = Qx.size();
+    double* dH = dev_alloc<double>(n);
+    double* dZ = dev_alloc<double>(n);
+    double* dQx = dev_alloc<double>(nq);
+    double* dQy = dev_alloc<double>(nq);
+    double* dVx = dev_alloc<double>(n);
+    double* dVy = dev_alloc<double>(n);
+    hip_ok(hipMemcpy(dH, H.data(), n * 8, hipMemcpyHostToDevice), "copy H");
+    hip_ok(hipMemcpy(dZ, dem.data(), n * 8, hipMemcpyHostToDevice), "copy DEM");
+    hip_ok(hipMemcpy(dQx, Qx.data(), nq * 8, hipMemcpyHostToDevice), "copy Qx");
+    hip_ok(hipMemcpy(dQy, Qy.data(), nq * 8, hipMemcpyHostToDevice), "copy Qy");
+    ComputeVelocity_GPU(dVx, dVy, dH, dZ, dQx, dQy, out.g.nx, out.g.ny, dx, out.g.nodata);
+    std::vector<double> vx(n), vy(n);
+    hip_ok(hipMemcpy(vx.data(), dVx, n * 8, hipMemcpyDeviceToHost), "copy Vx");
+    hip_ok(hipMemcpy(vy.data(), dVy, n * 8, hipMemcpyDeviceToHost), "copy Vy");
+    for (void* p : {(void*)dH, (void*)dZ, (void*)dQx, (void*)dQy, (void*)dVx, (void*)dVy})
+        (void)hipFree(p);
+    out.asc("Vx_final.asc", vx);
+    out.asc("Vy_final.asc", vy);
+}
+
+struct Result {
+    double t = 0.0;
+    long long steps = 0;
+    std::vector<double> H, Qx, Qy;
+};
+
+// The reference's time loop through the drop-in wrappers (main.cpp:196-324).
+Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& out) {
+    const int nx = d.nx, ny = d.ny;
+    const size_t n = (size_t)nx * ny, nq = (size_t)(nx + 1) * (ny + 1);
+    const double dx = d.dx, dA = dx * dx;
+    double* H = dev_alloc<double>(n);
+    double* DEM = dev_alloc<double>(n);
+    double* Qx = dev_alloc<double>(nq);
+    double* Qy = dev_alloc<double>(nq);
+    double* Qxold = dev_alloc<double>(nq);
+    double* Qyold = dev_alloc<double>(nq);
+    void* ws = dev_alloc<char>(AF_FLOODPLAINQ_WORKSPACE_BYTES);
+    hip_ok(hipMemcpy(DEM, d.dem, n * 8, hipMemcpyHostToDevice), "copy DEM");
+    if (d.h0) hip_ok(hipMemcpy(H, d.h0, n * 8, hipMemcpyHostToDevice), "copy H0");
+    af::compat::bind(nullptr, ws);
+
+    // flow (QFIX/QVAR) and other (FREE/HFIX/HVAR) points, main.cpp:77-85
+    std::vector<int> flow, other;
+    for (int p = 0; p < d.n_points; p++) {
+        const int ty = d.pt_type[p];
+        if (ty == AF_BC_QFIX || ty == AF_BC_QVAR) flow.push_back(p);
+        else other.push_back(p);
+    }
+    auto idx_of = [&](int p) { return d.pt_x[p] + d.pt_y[p] * nx; };
+    auto value_at = [&](int p, double t) {
+        const int len = d.pt_series_len ? d.pt_series_len[p] : 0;
+        const int off = d.pt_series_off ? d.pt_series_off[p] : 0;
+        return af_interp_series(d.series_t + off, d.series_v + off, len, d.pt_value[p], t);
+    };
+    const int nf = (int)flow.size(), no = (int)other.size();
+    std::vector<int32_t> ps_idx_h(nf), bc_idx_h(no), bc_type_h(no);
+    std::vector<double> ps_flow_h(nf), bc_value_h(no);
+    for (int s = 0; s < nf; s++) ps_idx_h[s] = idx_of(flow[s]);
+    for (int b = 0; b < no; b++) {
+        const int p = other[b], ty = d.pt_type[p];
+        bc_idx_h[b] = idx_of(p);
+        bc_type_h[b] = ty == AF_BC_FREE ? 1 : ty == AF_BC_HFIX ? 2 : ty == AF_BC_HVAR ? 3 : 0;
+        bc_value_h[b] = d.pt_value[p];
+    }
+    int32_t* ps_idx = dev_alloc<int32_t>(nf);
+    double* ps_flow = dev_alloc<double>(nf);
+    int32_t* bc_idx = dev_alloc<int32_t>(no);
+    int32_t* bc_type = dev_alloc<int32_t>(no);
+    double* bc_value = dev_alloc<double>(no);
+    if (nf) hip_ok(hipMemcpy(ps_idx, ps_idx_h.data(), nf * 4, hipMemcpyHostToDevice), "ps_idx");
+    if (no) {
+        hip_ok(hipMemcpy(bc_idx, bc_idx_h.data(), no * 4, hipMemcpyHostToDevice), "bc_idx");
+        hip_ok(hipMemcpy(bc_type, bc_type_h.data(), no * 4, hipMemcpyHostToDevice), "bc_type");
+    }
+
+    std::vector<double> Hh(n);
+    auto pull_H = [&]() { hip_ok(hipMemcpy(Hh.data(), H, n * 8, hipMemcpyDeviceToHost), "H"); };
+    int save_num = 0;
+    char name[64];
+    pull_H();
+    snprintf(name, sizeof name, "H_%04d.asc", save_num++);
+    out.asc(name, Hh, " (initial state)");
+
+    double t = 0.0, next_save = par.save_int, next_mass = par.mass_int;
+    long long step = 0;
+    while (t < d.sim_time) {
+        double Tstep = d.init_tstep;
+        FloodplainQ_GPU(H, DEM, Qx, Qy, Qxold, Qyold, nx, ny, dx, dx, d.fpn, d.depth_thresh,
+                        d.max_hflow, 0.01, Tstep, 1.0, dA, d.cfl, d.g, d.init_tstep, d.theta,
+                        d.nodata);
+        BCs_GPU(Qx, Qy, H, DEM, nx, ny, dx, d.g, Tstep, d.nodata, d.west_free, d.east_free,
+                d.north_free, d.south_free);
+        if (!out.quiet && (step < 10 || step % 100 == 0)) {
+            printf("Step %lld, t=%.2fs, dt=%.4fs\n", step, t, Tstep);
+            fflush(stdout);
+        }
+        if (nf) {
+            for (int s = 0; s < nf; s++) ps_flow_h[s] = value_at(flow[s], t) * dx * Tstep / dA;
+            hip_ok(hipMemcpy(ps_flow, ps_flow_h.data(), nf * 8, hipMemcpyHostToDevice), "ps_flow");
+            AddFlowBCs_GPU(H, ps_idx, ps_flow, nf);
+        }
+        if (d.rain_enabled) {
+            const double r = af_interp_series(d.rain_t, d.rain_r, d.n_rain, 0.0, t);
+            Rainfall_GPU(H, DEM, nx, ny, r, Tstep, d.nodata);
+        }
+        UpdateH_GPU(H, Qx, Qy, nx, ny, dA, Tstep);
+        if (no) {
+            for (int b = 0; b < no; b++)
+                if (d.pt_type[other[b]] == AF_BC_HVAR) bc_value_h[b] = value_at(other[b], t);
+            hip_ok(hipMemcpy(bc_value, bc_value_h.data(), no * 8, hipMemcpyHostToDevice),
+                   "bc_value");
+            ApplyPointBCs_GPU(H, bc_idx, bc_type, bc_value, no, nx, dx, dA, Tstep);
+        }
+        UpdateQs_GPU(Qx, Qy, Qxold, Qyold, nx, ny, dx);
+        t += Tstep;
+        step++;
+        if (t >= next_mass) {
+            pull_H();
+            double vol, area;
+            mass_of(Hh, dA, &vol, &area);
+            report_line(t, Tstep, vol, area);
+            next_mass += d.mass_int;
+        }
+        if (t >= next_save) {
+            pull_H();
+            snprintf(name, sizeof name, "H_%04d.asc", save_num++);
+            out.asc(name, Hh);
+            next_save += par.save_int;
+        }
+    }
+    Result r;
+    r.t = t;
+    r.steps = step;
+    pull_H();
+    r.H = Hh;
+    r.Qx.resize(nq);
+    r.Qy.resize(nq);
+    hip_ok(hipMemcpy(r.Qx.data(), Qx, nq * 8, hipMemcpyDeviceToHost), "Qx");
+    hip_ok(hipMemcpy(r.Qy.data(), Qy, nq * 8, hipMemcpyDeviceToHost), "Qy");
+    for (void* p : {(void*)H, (void*)DEM, (void*)Qx, (void*)Qy, (void*)Qxold, (void*)Qyold, ws,
+                    (void*)ps_idx, (void*)ps_flow, (void*)bc_idx, (void*)bc_type, (void*)bc_value})
+        (void)hipFree(p);
+    return r;
+}
+
+// The fused domain: one launch per step, graph-replayed, stopping at each
+// save time.  Mass reports are recorded on the device at the exact steps.
+Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out, int precision) {
+    const size_t n = (size_t)d.nx * d.ny, nq = (size_t)(d.nx + 1) * (d.ny + 1);
+    af_tuning tun{};
+    const double nrep = d.mass_int > 0 ? std::ceil(d.sim_time / d.mass_int) + 4 : 4;
+    tun.report_cap = (int32_t)std::min(nrep, 1048576.0);
+    size_t bytes = 0;
+    af_ok(af_domain_arena_bytes(&d, precision, &tun, &bytes), "arena_bytes");
+    void* arena = nullptr;
+    hip_ok(hipMalloc(&arena, bytes), "hipMalloc arena");
+    af_domain* D = nullptr;
+    af_ok(af_domain_create(&d, precision, &tun, arena, bytes, nullptr, &D), "domain_create");
+
+    std::vector<double> Hh(n), rep;
+    int save_num = 0, shown = 0;
+    char name[64];
+    af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
+    snprintf(name, sizeof name, "H_%04d.asc", save_num++);
+    out.asc(name, Hh, " (initial state)");
+
+    auto show_reports = [&]() {
+        af_run_info info;
+        af_ok(af_domain_info(D, &info, nullptr), "info");
+        if (info.n_reports <= shown) return;
+        rep.assign(4 * (size_t)info.n_reports, 0.0);
+        int32_t got = 0;
+        af_ok(af_domain_read_reports(D, rep.data(), info.n_reports, &got, nullptr), "reports");
+        for (int i = shown; i < got; i++)
+            report_line(rep[4 * i], rep[4 * i + 1], rep[4 * i + 2], rep[4 * i + 3]);
+        shown = got;
+    };
+
+    af_run_info info;
+    af_ok(af_domain_info(D, &info, nullptr), "info");
+    double next_save = par.save_int;
+    while (info.t < d.sim_time) {
+        const double stop = (par.save_int > 0 && next_save < d.sim_time) ? next_save : d.sim_time;
+        if (info.t >= stop) {
+            // one step jumped past two save times: the reference saves again
+            // after the next step
+            af_ok(af_batch_set_sim_time(D, 0, d.sim_time, nullptr), "set_sim_time");
+            af_ok(af_domain_advance(D, 1, 0, nullptr), "advance");
+        } else {
+            af_ok(af_batch_set_sim_time(D, 0, stop, nullptr), "set_sim_time");
+            af_ok(af_domain_run(D, 216, 1, nullptr), "run");
+        }
+        af_ok(af_domain_info(D, &info, nullptr), "info");
+        show_reports();
+        if (info.t >= next_save) {
+            af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
+            snprintf(name, sizeof name, "H_%04d.asc", save_num++);
+            out.asc(name, Hh);
+            next_save += par.save_int;
+        }
+    }
+    Result r;
+    r.t = info.t;
+    r.steps = info.steps_done;
+    r.H.resize(n);
+    r.Qx.resize(nq);
+    r.Qy.resize(nq);
+    af_ok(af_domain_read(D, 0, r.H.data(), nullptr), "read H");
+    af_ok(af_domain_read(D, 1, r.Qx.data(), nullptr), "read Qx");
+    af_ok(af_domain_read(D, 2, r.Qy.data(), nullptr), "read Qy");
+    af_domain_destroy(D);
+    (void)hipFree(arena);
+    return r;
+}
+
+int usage(const char* argv0) {
+    fprintf(stderr, "usage: %s [--path compat|fp64|fp32] [--quiet] <parfile>\n", argv0);
+    return 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string path = "compat";
+    const char* parfile = nullptr;
+    bool quiet = false;
+    for (int i = 1; i < argc; i++) {
+        if (!strcmp(argv[i], "--path") && i + 1 < argc) path = argv[++i];
+        else if (!strcmp(argv[i], "--quiet")) quiet = true;
+        else if (argv[i][0] == '-') return usage(argv[0]);
+        else parfile = argv[i];
+    }
+    if (!parfile || (path != "compat" && path != "fp64" && path != "fp32")) return usage(argv[0]);
+    af_case* c = af_case_new();
+    try {
+        printf("=== LISFLOOD-GPU Flood Simulation (MI355X, %s path) ===\n", path.c_str());
+        printf("Reading parameter file: %s\n", parfile);
+        af_ok(af_case_load(c, parfile), "load");
+        af_par par;
+        af_ok(af_case_par(c, &par), "par");
+        af_scenario_desc d;
+        af_ok(af_case_scenario(c, &d), "scenario");
+        Output out;
+        out.quiet = quiet;
+        out.dir = par.output_dir;
+        af_ok(af_case_dem_header(c, &out.g.nx, &out.g.ny, &out.g.xll, &out.g.yll, &out.g.cell,
+                                 &out.g.nodata),
+              "dem header");
+        printf("Grid: %d x %d, cellsize %.3f, sim time %.0f s, save every %.0f s\n", d.nx, d.ny,
+               d.dx, d.sim_time, par.save_int);
+        mkdir(par.output_dir, 0755);
+        const auto w0 = std::chrono::steady_clock::now();
+        Result r = path == "compat" ? run_compat(d, par, out)
+                                    : run_fused(d, par, out, path == "fp64" ? AF_FP64 : AF_FP32);
+        const double wall =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+        out.asc("H_final.asc", r.H);
+        if (par.voutput) {
+            std::vector<double> dem(d.dem, d.dem + (size_t)d.nx * d.ny);
+            velocity_files(out, r.H, dem, r.Qx, r.Qy, d.dx);
+        }
+        double vol, area;
+        mass_of(r.H, d.dx * d.dx, &vol, &area);
+        printf("\n=== Simulation Complete ===\n");
+        printf("Simulation time: %.0f s, Steps: %lld\n", r.t, r.steps);
+        printf("Wall clock time: %.2f s (%.2f min)\n", wall, wall / 60.0);
+        printf("Final volume: %.0f m3\n", vol);
+        printf("Output saved to %s/\n", par.output_dir);
+    } catch (const Fail& f) {
+        fprintf(stderr, "lisflood_amd: %s\n", f.msg.c_str());
+        af_case_free(c);
+        return 1;
+    } catch (const af::compat::error& e) {
+        fprintf(stderr, "lisflood_amd: %s\n", e.what());
+        af_case_free(c);
+        return 1;
+    }
+    af_case_free(c);
+    return 0;
+}

@@ -485,6 +485,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     c.hmax[(hs + 2) % 3] = host_key(-DBL_MAX);
     c.t[cur ^ 1] = 0.0;
     c.next_mass[cur ^ 1] = d->mass_int;
+    c.par[0] = c.par[1] = parity;  // the initial state sits in buffer `parity`
     if ((rc = up(h.o_ctl, &c, sizeof(c)))) return rc;
     AF_HIP(hipStreamSynchronize(st));
     return AF_OK;
@@ -933,6 +934,19 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
         int rc = af_domain_advance(D, check_every, use_graph, stream);
         if (rc) return rc;
     }
+    return AF_OK;
+}
+
+int af_batch_set_sim_time(af_domain* D, int s, double sim_time, af_stream_t stream) {
+    AF_REQUIRE(D && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
+               "set_sim_time: bad arguments");
+    AF_REQUIRE(sim_time == sim_time, AF_E_INVALID, "set_sim_time: NaN");
+    hipStream_t st = to_stream(stream);
+    D->scn[s].dev.sim_time = sim_time;
+    // the kernel reads the table on every launch, graph replays included
+    AF_HIP(hipMemcpyAsync(&D->d_scns[s].sim_time, &D->scn[s].dev.sim_time, sizeof(double),
+                          hipMemcpyHostToDevice, st));
+    AF_HIP(hipStreamSynchronize(st));
     return AF_OK;
 }
 

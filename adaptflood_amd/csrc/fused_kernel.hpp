@@ -39,6 +39,7 @@ struct Ctl {
     int pending_buf;              // which partial buffer holds them
     double pending_t, pending_dt;
     int n_reports;
+    int par[2];                   // buffer holding the state entering step k: par[k & 1]
     int pad_;
 };
 
@@ -459,6 +460,9 @@ struct Stepper {
         gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         const int tile_local = td.strip + td.seg * S.nstrips;
 
+        // state buffers: a skipped step leaves the state where it is, so the
+        // buffer pair follows the scenario's own executed steps
+        const int s_in = ctl->par[cur], s_out = s_in ^ 1;
         const double t = ctl->t[cur];
         const double next_mass = ctl->next_mass[cur];
         const double hmax_in = fkey_inv_d(ctl->hmax[hs_cur]);
@@ -466,8 +470,8 @@ struct Stepper {
         const double dt = cfl_dt(hmax_in, S.thresh, S.cfl, S.dx, S.g, S.init_tstep);
         const bool report_due = !skip && (t + dt >= next_mass);
 
-        if (td.first) bookkeeping(S, cur, nxt, hs_cur, hs_nxt, hs_rst, t, dt, next_mass, skip,
-                                  report_due, lane);
+        if (td.first) bookkeeping(S, cur, nxt, hs_cur, hs_nxt, hs_rst, s_in, t, dt, next_mass,
+                                  skip, report_due, lane);
         if (skip) return;
 
         TileCtx<T> x;
@@ -481,12 +485,12 @@ struct Stepper {
         x.lane = lane;
         x.nod = (T)S.nodata_t;
         x.z = glob<const T>(S.z);
-        x.Ho = glob<const T>(S.H[cur]);
-        x.Hn = glob<T>(S.H[nxt]);
-        x.Qxo = glob<const T>(S.Qx[cur]);
-        x.Qxn = glob<T>(S.Qx[nxt]);
-        x.Qyo = glob<const T>(S.Qy[cur]);
-        x.Qyn = glob<T>(S.Qy[nxt]);
+        x.Ho = glob<const T>(S.H[s_in]);
+        x.Hn = glob<T>(S.H[s_out]);
+        x.Qxo = glob<const T>(S.Qx[s_in]);
+        x.Qxn = glob<T>(S.Qx[s_out]);
+        x.Qyo = glob<const T>(S.Qy[s_in]);
+        x.Qyn = glob<T>(S.Qy[s_out]);
         x.sc = (T)S.dxinv;  // Q -> q_old (kernels.cpp:8-16)
         x.dxT = (T)S.dx;
         x.gT = (T)S.g;
@@ -637,15 +641,16 @@ struct Stepper {
     // finalise the previous step's mass report, advance t, rotate the H-max
     // slots, log dt.
     static __device__ __forceinline__ void bookkeeping(const ScnDev& S, int cur, int nxt, int hs_cur, int hs_nxt,
-                                       int hs_rst, double t, double dt, double next_mass,
-                                       bool skip, bool report_due, int lane) {
+                                       int hs_rst, int s_in, double t, double dt,
+                                       double next_mass, bool skip, bool report_due, int lane) {
         gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         finalize_report(S, lane);
         if (lane == 0) {
+            ctl->hmax[hs_rst] = key_neg_max();  // the slot step k+1 max-reduces into
             if (!skip) {
                 const double t_new = t + dt;  // main.cpp:287
                 ctl->t[nxt] = t_new;
-                ctl->hmax[hs_rst] = key_neg_max();
+                ctl->par[nxt] = s_in ^ 1;
                 const long long n = ctl->n_done;
                 if (n < S.dt_log_cap) glob<double>(S.dt_log)[n] = dt;
                 ctl->n_done = n + 1;
@@ -661,6 +666,7 @@ struct Stepper {
                 }
             } else {
                 ctl->t[nxt] = t;
+                ctl->par[nxt] = s_in;
                 ctl->next_mass[nxt] = next_mass;
                 ctl->hmax[hs_nxt] = ctl->hmax[hs_cur];
             }

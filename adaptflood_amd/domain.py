@@ -99,6 +99,11 @@ class Batch:
         return {"t": r.t, "steps": r.steps_done, "last_dt": r.last_dt,
                 "n_reports": r.n_reports, "finished": bool(r.finished), "hmax": r.hmax}
 
+    def set_sim_time(self, sim_time, i=0):
+        """Moves the stop time of scenario i (runs stop at the first step with
+        t >= sim_time, as the reference's loop does)."""
+        call("af_batch_set_sim_time", self.handle, i, float(sim_time), self._s())
+
     def read(self, what="H", i=0):
         """State in the reference layout as float64: H (ny, nx); Qx, Qy,
         Qxold, Qyold ((ny+1)*(nx+1) flat, pitch nx+1)."""

@@ -227,6 +227,11 @@ int af_batch_size(af_domain* d, int* n_scenarios, int* n_tiles);
 /* *all_done = 1 when every scenario has reached its sim_time. */
 int af_batch_finished(af_domain* d, int* all_done, af_stream_t stream);
 int af_batch_info(af_domain* d, int scenario, af_run_info* info, af_stream_t stream);
+/* Moves the time at which `scenario` stops (the loop condition t < sim_time
+ * of src/main.cpp:227).  A caller that stops at each save time, writes its
+ * output and moves the stop time on reproduces the reference's save cadence
+ * (main.cpp:327): the run stops at the first step with t >= the stop time. */
+int af_batch_set_sim_time(af_domain* d, int scenario, double sim_time, af_stream_t stream);
 int af_batch_read(af_domain* d, int scenario, int which, double* out, af_stream_t stream);
 int af_batch_read_dt_log(af_domain* d, int scenario, double* out, int64_t cap, int64_t* n,
                          af_stream_t stream);

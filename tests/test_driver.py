@@ -1,0 +1,29 @@
+"""The command-line driver (lisflood_amd) without a GPU: argument handling
+and input errors end cleanly before any device call."""
+import os
+import subprocess
+
+import pytest
+
+EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "adaptflood_amd",
+                   "_lib", "lisflood_amd")
+
+
+@pytest.fixture(scope="module")
+def exe():
+    if not os.path.exists(EXE):
+        pytest.fail("lisflood_amd is not built (run __graft_entry__.build())")
+    return EXE
+
+
+def test_usage(exe):
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "--path", "fp16", "x.par"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+
+
+def test_missing_par_file(exe, tmp_path):
+    r = subprocess.run([exe, str(tmp_path / "none.par")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "lisflood_amd: load" in r.stderr

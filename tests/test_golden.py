@@ -1,0 +1,62 @@
+"""The oracle against the committed golden vectors of the 256 x 256 plane
+workload (tests/golden/plane256.npz, written by tests/golden/make_golden.py),
+plus invariants the vectors themselves must satisfy."""
+import numpy as np
+
+from adaptflood_amd import generators
+from oracle import oracle
+
+from golden_data import dem_digest, load
+
+
+def test_generator_still_makes_the_golden_dem():
+    g = load()
+    assert dem_digest(generators.plane_case().dem) == str(g["dem_sha256"])
+
+
+def test_oracle_reproduces_golden_trajectory():
+    g = load()
+    r = oracle.OracleRun(generators.plane_case())
+    done = 0
+    for k in (1, 10, 100):
+        r.advance(k - done)
+        done = k
+        st = r.state()
+        assert np.array_equal(st["H"], g[f"H_{k}"]), k
+        assert st["t"] == float(g[f"t_{k}"])
+    assert np.array_equal(r.Qx, g["Qx_100"])
+    assert np.array_equal(r.Qy, g["Qy_100"])
+    r.advance()
+    st = r.state()
+    assert st["steps"] == int(g["steps_end"])
+    assert st["t"] == float(g["t_end"])
+    assert np.array_equal(np.array(st["dt"]), g["dt"])
+    assert np.array_equal(st["H"], g["H_end"])
+    assert np.array_equal(st["Qx"], g["Qx_end"])
+    np.testing.assert_allclose(st["reports"], g["reports"], rtol=1e-12, atol=0)
+
+
+def test_golden_invariants():
+    """What the physics fixes about the stored trajectory: one step from dry
+    adds exactly v*dx*dt to the source cell (dt = InitTstep = 10 s, the only
+    wet cell); t is the running sum of dt; reports fall on the mass interval
+    and the depth is nonnegative everywhere.  (The reported volume exceeds the
+    injected 5 m^3/s * t: the clamp H = max(H, 0) of the scheme adds water
+    wherever a cell's outflow overdraws it; test_oracle's closed-domain test
+    accounts for that gain exactly.)"""
+    g = load()
+    scn = generators.plane_case()
+    H1 = g["H_1"]
+    src = scn.points[0]
+    expect = src.value * scn.dx * 10.0 / (scn.dx * scn.dx)
+    assert H1[src.y, src.x] == expect
+    assert np.count_nonzero(H1) == 1
+    assert float(g["t_1"]) == 10.0
+    assert float(np.cumsum(g["dt"])[-1]) == float(g["t_end"])  # sequential sum, as t += dt
+    rep = g["reports"]
+    assert np.all(rep[:, 2] > 0)
+    k = np.arange(1, len(rep) + 1) * scn.mass_int
+    assert np.all(rep[:, 0] >= k) and np.all(rep[:, 0] - rep[:, 1] < k)
+    assert np.all(g["H_end"] >= 0)
+    assert float(g["t_end"]) >= scn.sim_time
+    assert float(g["t_end"]) - float(g["dt"][-1]) < scn.sim_time

@@ -333,3 +333,39 @@ def test_step_count_and_time_sequence():
     assert out["t"] >= 95.0 and out["t"] - out["dt"][-1] < 95.0
     assert out["dt"][0] == 10.0          # dry start
     assert out["dt"][1] < 10.0           # the source wetted a cell above the threshold
+
+
+# ---------------------------------------------------------- velocity (8f-1)
+def test_velocity_kat():
+    """3 x 2 grid, dx = 2, hand-computed: each cell reports the face velocity
+    Q/dx/h of larger magnitude among its two faces; |Q| <= 1e-10 counts as 0;
+    NODATA cells and cells shallower than 1 mm report 0."""
+    nx, ny, dx = 3, 2, 2.0
+    H = np.array([0.5, 1.0, 0.0005,
+                  2.0, 0.25, 1.0])
+    DEM = np.array([0.0, 0.0, 0.0,
+                    0.0, 0.0, NOD])
+    P = nx + 1
+    Qx = np.zeros(P * (ny + 1))
+    Qy = np.zeros(P * (ny + 1))
+    # row 0 faces: f0 = 0.2, f1 = -1.0, f2 = 0.3, f3 = 5e-11
+    Qx[0:4] = [0.2, -1.0, 0.3, 5e-11]
+    # row 1 faces: f0 = 0, f1 = 0.4, f2 = -0.2, f3 = 9.0
+    Qx[4:8] = [0.0, 0.4, -0.2, 9.0]
+    # y faces: row g = 0 (north edge), 1, 2 (south edge)
+    Qy[0:3] = [1e-11, 0.6, 0.0]
+    Qy[4:7] = [-0.4, 0.2, 0.5]
+    Qy[8:11] = [0.8, -0.1, 3.0]
+    Vx, Vy = oracle.velocity(H, DEM, Qx, Qy, nx, ny, dx, NOD)
+    # cell (0,0): h .5: left .2/2/.5 = .2, right -1/2/.5 = -1 -> -1
+    # cell (1,0): h 1: left -.5, right .15 -> -.5
+    # cell (2,0): h < 1 mm -> 0
+    # cell (0,1): h 2: left 0, right .4/2/2 = .1 -> .1
+    # cell (1,1): h .25: left .8, right -.4 -> .8
+    # cell (2,1): NODATA -> 0
+    np.testing.assert_allclose(Vx, [-1.0, -0.5, 0.0, 0.1, 0.8, 0.0], rtol=1e-15, atol=0)
+    # cell (0,0): top 1e-11 -> 0, bottom -.4/2/.5 = -.4 -> -.4
+    # cell (1,0): top .6/2/1 = .3, bottom .2/2/1 = .1 -> .3
+    # cell (0,1): top -.4/2/2 = -.1, bottom .8/2/2 = .2 -> .2
+    # cell (1,1): top .2/2/.25 = .4, bottom -.1/2/.25 = -.2 -> .4
+    np.testing.assert_allclose(Vy, [-0.4, 0.3, 0.0, 0.2, 0.4, 0.0], rtol=1e-15, atol=0)
