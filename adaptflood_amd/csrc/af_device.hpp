@@ -15,6 +15,43 @@ __host__ __device__ __forceinline__ T getmax(T a, T b) { return (a > b) ? a : b;
 template <typename T>
 __host__ __device__ __forceinline__ T getmin(T a, T b) { return (a < b) ? a : b; }
 
+// pow(h, 10.0/3.0) for h > 0 (the friction term of kernels.cpp:83 and :133),
+// built from IEEE operations only (+, -, *, fma, frexp, ldexp), so it
+// rounds the same way on every machine that rounds those correctly.
+//   h = xr * 2^(3q), xr in [1, 8):  h^(10/3) = xr^3 * cbrt(xr) * 2^(10q).
+//   cbrt(xr): Newton's iteration for the inverse cube root y = xr^(-1/3),
+//     y <- y (4 - xr y^3) / 3, from a quadratic start (6 steps), then one
+//     Newton correction of c0 = xr y^2 with the residual xr - c0^3 taken in
+//     double-double (Dekker's exact products through fma).
+//   xr^3 in double-double, times cbrt as a double-double product.
+//   The exponent 10.0/3.0 exceeds 10/3 by 1.4802973661668753e-16; that
+//     factor h^1.48e-16 = 1 + 1.48e-16 ln h is applied to first order (ln h
+//     from a cubic fit on [1, 2) is ample: the term is below 5 ulp).
+// The result is the correctly rounded pow(h, 10.0/3.0) but for about 3 in
+// 10^4 arguments, which are 1 ulp away; it differs from glibc's pow in about
+// 1 in 10^3 arguments, by 1 ulp.
+__host__ __device__ __forceinline__ double pow_10_3(double h) {
+    int E;
+    const double f = frexp(h, &E);                       // h = f 2^E, f in [0.5, 1)
+    const int e = E - 1;
+    const int q = (e >= 0) ? e / 3 : -((2 - e) / 3);     // floor(e / 3)
+    const double xr = ldexp(2.0 * f, e - 3 * q);         // [1, 8)
+    double y = 1.190645532750796 + xr * (-0.20554765291607405 + xr * 0.01490212016527807);
+    for (int i = 0; i < 6; i++) y = y * (4.0 - xr * (y * y * y)) * (1.0 / 3.0);
+    const double c0 = xr * (y * y);
+    const double p = c0 * c0, pe = fma(c0, c0, -p);
+    const double t = p * c0, te = fma(p, c0, -t) + pe * c0;
+    const double c1 = ((xr - t) - te) * (y * y) * (1.0 / 3.0);
+    const double a = xr * xr, ae = fma(xr, xr, -a);
+    const double b = a * xr, be = fma(a, xr, -b) + ae * xr;
+    const double hi = b * c0, lo = fma(b, c0, -hi) + (b * c1 + be * c0);
+    const double u = 2.0 * f - 1.0;
+    const double lnh = e * 0.6931471805599453 +
+                       u * (0.9866915272798602 + u * (-0.40741277789409874 + u * 0.11478007077615415));
+    const double d = hi * (1.4802973661668753e-16 * lnh);
+    return ldexp(hi + (lo + d), 10 * q);
+}
+
 // Constants of one step's face fluxes, FP64 reference form.
 struct FluxConstsF64 {
     double dx;      // face width and the Sf divisor
@@ -43,7 +80,7 @@ __device__ __forceinline__ double face_flux_ref(double z0, double z1, double h0,
     if (hflow > k.thresh) {
         double qvect = sqrt(q0 * q0 + qt * qt);
         double num = (k.theta * q0 + 0.5 * (1.0 - k.theta) * (qup + qdown)) - (k.gdt * hflow * Sf);
-        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow_10_3(hflow);
         Q = num / den * k.dx;
         if (Q * dh < 0.0) {
             num = q0 - (k.gdt * hflow * Sf);
@@ -69,7 +106,7 @@ __device__ __forceinline__ double face_flux_ref_t1(double z0, double z1, double 
         double Sf = -dh / k.dx;
         double qvect = sqrt(q0 * q0 + qt * qt);
         double num = (k.theta * q0) - (k.gdt * hflow * Sf);
-        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow_10_3(hflow);
         Q = num / den * k.dx;
     }
     return Q;
@@ -94,7 +131,7 @@ __device__ __forceinline__ double face_flux_ref_t1_q(double z0, double z1, doubl
         double Sf = -dh / k.dx;
         double qvect = sqrt(q0 * q0 + qt * qt);
         double num = (k.theta * q0) - (k.gdt * hflow * Sf);
-        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow(hflow, 10.0 / 3.0);
+        double den = 1.0 + k.gdt * hflow * k.fn * k.fn * fabs(qvect) / pow_10_3(hflow);
         Q = num / den * k.dx;
     }
     return Q;

@@ -38,6 +38,7 @@ struct HostScn {
     int nx = 0, ny = 0;
     long long P = 0;
     int nstrips = 0, nsegs = 0, ntiles = 0, tile_base = 0;
+    int R = 0;                   // tile rows of this scenario
     double zmin = 0.0;
     // offsets in the arena
     size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
@@ -63,6 +64,7 @@ struct HostScn {
 struct af_domain {
     int precision = AF_FP32;
     int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36, variant = 0;
+    int R_fixed = 0;             // tile rows asked for (0: chosen per scenario)
     long long dt_log_cap = 0;
     int report_cap = 64;
     char* arena = nullptr;
@@ -86,6 +88,7 @@ namespace af {
 namespace {
 
 constexpr int kVolBlocks = 1024;
+constexpr int kMinTiles = 1024;   // wavefronts a lone scenario should offer
 
 int elem_size(int precision) { return precision == AF_FP64 ? 8 : 4; }
 
@@ -154,7 +157,17 @@ void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     }
     h.P = (long long)(row / es);
     h.nstrips = (d->nx + D->W - 1) / D->W;
-    h.nsegs = (d->ny + D->R - 1) / D->R;
+    // Tile rows: the tuning's value, else the default height, halved while
+    // the scenario alone would give fewer than kMinTiles wavefronts (a small
+    // domain run by itself is latency-bound: short tiles keep more rows in
+    // flight at once).
+    int R = D->R_fixed;
+    if (R <= 0) {
+        R = D->R;
+        while (R > 8 && (long long)h.nstrips * ((d->ny + R - 1) / R) < kMinTiles) R /= 2;
+    }
+    h.R = R;
+    h.nsegs = (d->ny + R - 1) / R;
     h.ntiles = h.nstrips * h.nsegs;
 }
 
@@ -208,7 +221,7 @@ void plan_points(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     for (int p = 0; p < d->n_points; p++) {
         long long idx = (long long)d->pt_x[p] + (long long)d->pt_y[p] * d->nx;
         int j = (int)(idx / d->nx), i = (int)(idx % d->nx);
-        int tile = (i / D->W) + (j / D->R) * h.nstrips;
+        int tile = (i / D->W) + (j / h.R) * h.nstrips;
         int ty = d->pt_type[p];
         if (ty == AF_BC_QFIX || ty == AF_BC_QVAR) src.push_back({tile, j, i, p});
         else if (ty == AF_BC_FREE || ty == AF_BC_HFIX || ty == AF_BC_HVAR) ovr.push_back({tile, j, i, p});
@@ -257,7 +270,7 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
     D->dt_log_cap = 0;
     D->report_cap = 256;
     if (tun) {
-        if (tun->tile_rows > 0) D->R = tun->tile_rows;
+        if (tun->tile_rows > 0) D->R = D->R_fixed = tun->tile_rows;
         if (tun->waves_per_block > 0) D->wpb = tun->waves_per_block;
         if (tun->graph_steps > 0) D->graph_steps = tun->graph_steps;
         if (tun->dt_log_cap > 0) D->dt_log_cap = tun->dt_log_cap;
@@ -375,7 +388,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     v.ny = h.ny;
     v.P = h.P;
     v.nstrips = h.nstrips;
-    v.tile_rows = D->R;
+    v.tile_rows = h.R;
     v.tile_base = 0;
     v.ntiles = h.ntiles;
     v.z = A + h.o_z;

@@ -7,6 +7,7 @@ runs HIP kernels of libadaptflood_amd.so and returns after they finish, as
 the reference's functions do (each ends in Kokkos::fence()).
 """
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -159,6 +160,8 @@ def run_sequence(scn, max_steps=None, device="cuda"):
     bc_value_host = np.array([p.value for p in other] or [0.0], dtype=np.float64)
     rain = scn.rain_times is not None and len(scn.rain_times) > 0
     t, step = 0.0, 0
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter()
     next_mass = scn.mass_int
     dts, reports = [], []
     while t < scn.sim_time:
@@ -196,6 +199,8 @@ def run_sequence(scn, max_steps=None, device="cuda"):
             reports.append((t, Tstep, vol, area))
             next_mass += scn.mass_int
     torch.cuda.synchronize()
-    return {"H": H.cpu().numpy().reshape(ny, nx), "Qx": Qx.cpu().numpy(), "Qy": Qy.cpu().numpy(),
+    loop_seconds = time.perf_counter() - t_loop
+    return {"loop_seconds": loop_seconds,
+            "H": H.cpu().numpy().reshape(ny, nx), "Qx": Qx.cpu().numpy(), "Qy": Qy.cpu().numpy(),
             "Qxold": Qxold.cpu().numpy(), "Qyold": Qyold.cpu().numpy(), "t": t, "steps": step,
             "dt": np.array(dts), "reports": np.array(reports).reshape(-1, 4)}

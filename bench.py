@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -70,6 +72,313 @@ def measure(scn, precision, steps, warmup, tile_rows, wpb, variant=0):
     return wall, kern_s, info
 
 
+def _ensemble_member(k, seed_base, sim_time):
+    from adaptflood_amd import generators
+    return generators.ensemble_member(k, seed_base=seed_base, sim_time=sim_time)
+
+
+def generate_members(ks, seed_base, sim_times, workers=8):
+    """Ensemble members in worker processes (forked before any GPU use)."""
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_ensemble_member, ks, [seed_base] * len(ks), sim_times, chunksize=4))
+
+
+def _event_ms(torch, launch, n):
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(n):
+        launch()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def sequential_fused(scns, precision, steps, warmup=3):
+    """Each scenario alone through the fused single-domain path (graph-replayed),
+    back to back: aggregate cell-updates/s over the timed steps."""
+    import torch
+    from adaptflood_amd.domain import Domain, make_tuning
+    cu, sec = 0, 0.0
+    for scn in scns:
+        d = Domain(scn, precision=precision, tuning=make_tuning(graph_steps=36))
+        d.advance(warmup, graph=False)
+        d.advance(0, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.advance(steps, graph=True)
+        torch.cuda.synchronize()
+        sec += time.perf_counter() - t0
+        cu += scn.cells * steps
+        d.close()
+    return cu / sec / 1e9, sec
+
+
+def sequential_plain(scns, steps):
+    """Each scenario alone through the reference's call sequence (the compat
+    C ABI, FP64, one synchronous call per kernel, dt read back every step)."""
+    from adaptflood_amd import kernels
+    cu, sec = 0, 0.0
+    for scn in scns:
+        r = kernels.run_sequence(scn, max_steps=steps)
+        sec += r["loop_seconds"]
+        cu += scn.cells * r["steps"]
+    return cu / sec / 1e9, sec
+
+
+def bench_ensemble(args):
+    """512 ragged scenarios (sides log-uniform 128..2048, aspect 1..4) advanced
+    together, one launch per step, against the same scenarios run back to back
+    through the single-scenario paths."""
+    t = time.perf_counter()
+    n = args.members
+    scns = generate_members(list(range(n)), 4000, [3600.0] * n)
+    cells = sum(s.cells for s in scns)
+    print(f"[bench] generated {n} scenarios, {cells} cells, in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    import torch
+    from adaptflood_amd.domain import Batch, make_tuning
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    prec = args.precision
+    tun = make_tuning(graph_steps=36)
+    tuned = None
+    if args.autotune:
+        from adaptflood_amd.tune import autotune
+        t = time.perf_counter()
+        best, table = autotune(scns, prec, rows=(16, 32, 64, 128), launches=10,
+                               log=lambda m: print(f"[tune] {m}", file=sys.stderr, flush=True))
+        tun = make_tuning(tile_rows=best.tile_rows, waves_per_block=best.waves_per_block,
+                          variant=best.variant, graph_steps=36)
+        tuned = {"tile_rows": best.tile_rows, "variant": best.variant,
+                 "waves_per_block": best.waves_per_block, "tuning_s": round(time.perf_counter() - t, 1)}
+        print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
+    b = Batch(scns, precision=prec, tuning=tun)
+    b.advance(args.warmup, graph=False)
+    b.advance(0, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(args.steps, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = _event_ms(torch, b.launch_step, 20)
+    total = args.warmup + args.steps + 20
+    active = sum(1 for k in range(n) if b.info(k)["steps"] == total)
+    gcups = cells * args.steps / wall / 1e9
+    B = 28.0 if prec == "fp32" else 56.0
+    achieved = B * cells / (kern_ms * 1e-3) / 1e9
+    print(f"[bench] batched {prec}: {wall * 1e3 / args.steps:.3f} ms/step, {gcups:.1f} GCUPS, "
+          f"kernel {kern_ms:.3f} ms = {achieved:.0f} GB/s, {active}/{n} active",
+          file=sys.stderr, flush=True)
+    b.close()
+    fused_sample = scns[::args.seq_stride]
+    g_fused, s_fused = sequential_fused(fused_sample, prec, args.seq_steps)
+    print(f"[bench] sequential fused {prec} ({len(fused_sample)} scenarios): {g_fused:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    plain_sample = scns[::args.plain_stride]
+    g_plain, s_plain = sequential_plain(plain_sample, args.plain_steps)
+    print(f"[bench] sequential plain fp64 ({len(plain_sample)} scenarios): {g_plain:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "aggregate cell-updates/s, batched ensemble vs sequential single-scenario runs",
+        "value": round(gcups, 3), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"ensemble of {n} ragged scenarios (sides log-uniform 128..2048, "
+                               f"aspect U(1,4), dx 5 m, fractal DEM, 10-500 QVAR, rain, "
+                               f"own adaptive dt), one launch per step, graph-replayed",
+                   "scenarios": n, "cells": cells, "active_in_timed_steps": active,
+                   "precision": prec, "autotuned": tuned},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(kern_ms, 4)},
+        "sequential_fused": {"value": round(g_fused, 3), "unit": "GCUPS",
+                             "sample": f"every {args.seq_stride}th scenario "
+                                       f"({len(fused_sample)}), {args.seq_steps} steps each, "
+                                       f"{prec}, graph-replayed", "seconds": round(s_fused, 3)},
+        "sequential_plain": {"value": round(g_plain, 3), "unit": "GCUPS",
+                             "sample": f"every {args.plain_stride}th scenario "
+                                       f"({len(plain_sample)}), {args.plain_steps} steps each, "
+                                       f"fp64 reference call sequence", "seconds": round(s_plain, 3)},
+        "speedup_vs_sequential_fused": round(gcups / g_fused, 2),
+        "speedup_vs_sequential_plain": round(gcups / g_plain, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(args):
+    """Poisson arrivals (20/s) of ensemble scenarios with sim_time U(600, 3600) s,
+    admitted into a pool by HBM footprint, advanced together, released when done."""
+    from adaptflood_amd import generators
+    n = args.members
+    arr, sims = generators.arrivals(n, rate=args.rate, seed=5)
+    t = time.perf_counter()
+    scns = generate_members(list(range(n)), 5000, [float(x) for x in sims])
+    cells = sum(s.cells for s in scns)
+    print(f"[bench] generated {n} requests, {cells} cells, in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    import torch
+    from adaptflood_amd.serve import Pool, Request, serve
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    prec = args.precision
+    free, _ = torch.cuda.mem_get_info()
+    budget = min(int(args.budget_gb * (1 << 30)), int(free * 0.8))
+    pool = Pool(budget_bytes=budget, precision=prec, max_scenarios=n, max_tiles=1 << 20)
+    # a few requests through a warm pool first, so kernel loading is not timed
+    warm = [Request(-1 - k, generators.ensemble_member(k, seed_base=5000, sim_time=400.0), 0.0)
+            for k in range(2)]
+    serve(warm, pool, chunk_steps=36, clock="wall")
+    reqs = [Request(k, s, float(a)) for k, (s, a) in enumerate(zip(scns, arr))]
+    done, summary = serve(reqs, pool, chunk_steps=args.chunk, clock="wall")
+    print(f"[bench] served {summary['requests']} in {summary['wall_s']:.1f}s: "
+          f"{summary['gcups']:.1f} GCUPS, p50 {summary['p50_latency']:.2f}s, "
+          f"p99 {summary['p99_latency']:.2f}s", file=sys.stderr, flush=True)
+    pool.close()
+    del pool
+    torch.cuda.empty_cache()
+    sample = scns[::args.seq_stride]
+    g_fused, s_fused = sequential_fused(sample, prec, args.seq_steps)
+    print(f"[bench] sequential fused {prec} ({len(sample)} scenarios): {g_fused:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    plain_sample = scns[::args.plain_stride]
+    g_plain, s_plain = sequential_plain(plain_sample, args.plain_steps)
+    print(f"[bench] sequential plain fp64 ({len(plain_sample)} scenarios): {g_plain:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    steps = np.array([r.steps for r in done])
+    out = {
+        "metric": "served ensemble: p50/p99 scenario latency and aggregate cell-updates/s",
+        "value": round(summary["gcups"], 3), "unit": "GCUPS", "n_gpus": 1,
+        "steps": int(steps.sum()), "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"{n} requests, Poisson arrivals at {args.rate}/s, ensemble "
+                               f"generator (seed 5), sim_time U(600,3600) s, HBM budget "
+                               f"{budget / (1 << 30):.0f} GiB, chunks of {args.chunk} steps",
+                   "requests": n, "cells": cells, "precision": prec},
+        "p50_latency_s": round(summary["p50_latency"], 3),
+        "p99_latency_s": round(summary["p99_latency"], 3),
+        "wall_s": round(summary["wall_s"], 3),
+        "arrival_span_s": round(float(arr[-1]), 3),
+        "cell_updates": int(summary["cell_updates"]),
+        "sequential_fused": {"value": round(g_fused, 3), "unit": "GCUPS",
+                             "sample": f"every {args.seq_stride}th request ({len(sample)}), "
+                                       f"{args.seq_steps} steps each"},
+        "sequential_plain": {"value": round(g_plain, 3), "unit": "GCUPS",
+                             "sample": f"every {args.plain_stride}th request "
+                                       f"({len(plain_sample)}), {args.plain_steps} steps each, "
+                                       f"fp64 reference call sequence"},
+        "speedup_vs_sequential_fused": round(summary["gcups"] / g_fused, 2),
+        "speedup_vs_sequential_plain": round(summary["gcups"] / g_plain, 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _errors(H, ref, dA, wet=0.01):
+    """max and RMS |dH|, inundated-area difference (cells deeper than `wet`,
+    the reference's Area criterion, main.cpp:296) and relative mass error."""
+    d = np.abs(H - ref)
+    a, b = H > wet, ref > wet
+    vol, vref = float(np.sum(H)) * dA, float(np.sum(ref)) * dA
+    return {"max_abs_dH_m": float(d.max()), "rms_dH_m": float(np.sqrt(np.mean(d * d))),
+            "area_m2": float(a.sum()) * dA, "area_ref_m2": float(b.sum()) * dA,
+            "area_diff_m2": float(a.sum() - b.sum()) * dA,
+            "area_mismatch_cells": int(np.count_nonzero(a != b)),
+            "rel_mass_err": abs(vol - vref) / vref if vref else 0.0}
+
+
+def _full_run(scn, precision, label):
+    """Runs to sim_time in graph chunks, printing progress."""
+    import torch
+    from adaptflood_amd.domain import Domain, make_tuning
+    d = Domain(scn, precision=precision, tuning=make_tuning(graph_steps=216))
+    torch.cuda.synchronize()
+    t0 = last = time.perf_counter()
+    while True:
+        d.advance(2160, graph=True)
+        info = d.info()
+        if info["finished"]:
+            break
+        if time.perf_counter() - last > 20:
+            last = time.perf_counter()
+            print(f"[acc] {label} {precision}: t={info['t']:.0f}s steps={info['steps']} "
+                  f"({last - t0:.0f}s)", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    H = d.read("H")
+    d.close()
+    return H, info, wall
+
+
+def bench_accuracy(args):
+    """FP32 and FP64 against FP64 references.  The 4096^2 valley case to
+    --acc-time against the FP64 CPU oracle; the valley case to its full
+    7200 s and the 16384^2 roofline case to --acc-time at FP32 against the
+    FP64 GPU path (itself pinned to the oracle on the smaller runs)."""
+    import torch
+    from adaptflood_amd import generators
+    from oracle import oracle
+    lines = []
+    t = time.perf_counter()
+    scn = generators.valley_case(n=args.valley_size)
+    full_time = scn.sim_time
+    scn.sim_time = min(args.acc_time, full_time)
+    print(f"[acc] valley {scn.nx}x{scn.ny} generated in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    dA = scn.dx * scn.dx
+    t = time.perf_counter()
+    run = oracle.OracleRun(scn, dt_log_cap=1, mass_log_cap=1)
+    while run.t < scn.sim_time:
+        if run.advance(500) == 0:
+            break
+        print(f"[acc] oracle t={run.t:.0f}s steps={run.steps} ({time.perf_counter() - t:.0f}s)",
+              file=sys.stderr, flush=True)
+    ref = run.state()
+    t_orc = time.perf_counter() - t
+    for prec in ("fp64", "fp32"):
+        H, info, wall = _full_run(scn, prec, "valley")
+        e = _errors(H, ref["H"], dA)
+        e.update({"case": f"valley {scn.nx}x{scn.ny}, {scn.sim_time:.0f} s", "precision": prec,
+                  "against": "FP64 CPU oracle", "steps": info["steps"],
+                  "steps_ref": ref["steps"], "gpu_wall_s": round(wall, 3),
+                  "oracle_wall_s": round(t_orc, 1), "oracle_threads": oracle.num_threads()})
+        print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
+        lines.append(e)
+    del ref, run
+    scn.sim_time = full_time
+    H64, i64, w64 = _full_run(scn, "fp64", "valley")
+    H32, i32, w32 = _full_run(scn, "fp32", "valley")
+    e = _errors(H32, H64, dA)
+    e.update({"case": f"valley {scn.nx}x{scn.ny}, {full_time:.0f} s", "precision": "fp32",
+              "against": "FP64 GPU path", "steps": i32["steps"], "steps_ref": i64["steps"],
+              "gpu_wall_s": round(w32, 3), "ref_wall_s": round(w64, 3)})
+    print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
+    lines.append(e)
+    if args.size > 0:
+        t = time.perf_counter()
+        big = generators.roofline_case(n=args.size)
+        big.sim_time = args.acc_time
+        print(f"[acc] roofline {big.nx}x{big.ny} generated in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
+        H64, i64, w64 = _full_run(big, "fp64", "roofline")
+        H32, i32, w32 = _full_run(big, "fp32", "roofline")
+        e = _errors(H32, H64, big.dx * big.dx)
+        e.update({"case": f"roofline {big.nx}x{big.ny}, {big.sim_time:.0f} s",
+                  "precision": "fp32", "against": "FP64 GPU path", "steps": i32["steps"],
+                  "steps_ref": i64["steps"], "gpu_wall_s": round(w32, 3),
+                  "ref_wall_s": round(w64, 3)})
+        print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
+        lines.append(e)
+    torch.cuda.synchronize()
+    print(json.dumps({"metric": "FP32/FP64 error against FP64 reference paths",
+                      "results": lines}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,12 +395,37 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 4 GiB device-to-device copy (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--workload", choices=("roofline", "ensemble", "served", "accuracy"),
+                    default="roofline")
+    ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
+    ap.add_argument("--acc-time", type=float, default=600.0,
+                    help="accuracy: simulated seconds of the oracle and roofline comparisons")
+    ap.add_argument("--members", type=int, default=512, help="ensemble / served scenarios")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
+                    help="ensemble / served state precision")
+    ap.add_argument("--seq-stride", type=int, default=4,
+                    help="sequential fused baseline: every n-th scenario")
+    ap.add_argument("--seq-steps", type=int, default=72)
+    ap.add_argument("--plain-stride", type=int, default=16,
+                    help="sequential plain baseline: every n-th scenario")
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--rate", type=float, default=20.0, help="served arrivals per second")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="served HBM budget (GiB)")
+    ap.add_argument("--chunk", type=int, default=36, help="served steps per pool advance")
+    ap.add_argument("--autotune", action="store_true",
+                    help="ensemble: pick tile rows and kernel variant by timing candidates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "served":
+        return bench_served(args)
+    if args.workload == "accuracy":
+        return bench_accuracy(args)
     import torch
     from adaptflood_amd import generators
     if not torch.cuda.is_available():

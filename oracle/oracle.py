@@ -69,6 +69,8 @@ def lib():
         L.orc_dt_from_hmax.restype = D
         L.orc_face_flux.argtypes = [D] * 15
         L.orc_face_flux.restype = D
+        L.orc_pow_10_3.argtypes = [D]
+        L.orc_pow_10_3.restype = D
         L.orc_floodplain_q.argtypes = [V, V, V, V, V, V, I, I, D, D, D, D, D, D, _dp, D, D, D, D,
                                        D, D, D]
         L.orc_bcs.argtypes = [V, V, V, V, I, I, D, D, D, D, I, I, I, I]
@@ -260,6 +262,11 @@ def interp(times, values, empty_value, t):
 def face_flux(z0, z1, h0, h1, q0, qup, qdown, qt, dx, fn, thresh, maxh, dt, g, theta):
     return lib().orc_face_flux(z0, z1, h0, h1, q0, qup, qdown, qt, dx, fn, thresh, maxh, dt, g,
                                theta)
+
+
+def pow_10_3(h):
+    """The oracle's pow(h, 10.0/3.0) (IEEE-exact operations only)."""
+    return lib().orc_pow_10_3(float(h))
 
 
 def find_hmax(H, DEM, nodata):

@@ -2,7 +2,9 @@
 
 Bars: FP64 fused path max|dH| <= 1e-9 m against the oracle; FP32 fused path
 max|dH| <= 1e-3 m and relative volume error <= 1e-5.  Step counts and dt
-sequences must match the oracle's exactly at FP64.
+sequences must match the oracle's exactly at FP64.  The FP64 path evaluates
+the oracle's operations in the oracle's order (pow included), so its states
+are also checked to be bitwise equal.
 """
 import numpy as np
 import pytest
@@ -43,6 +45,10 @@ def test_fp64_matches_oracle_small(gpu, seed, nx, ny, rows):
     assert _faces_close(d.read("Qx"), ref["Qx"], 1e-8)
     assert _faces_close(d.read("Qy"), ref["Qy"], 1e-8)
     assert _faces_close(d.read("Qxold"), ref["Qxold"], 1e-9)
+    np.testing.assert_array_equal(d.read("H"), ref["H"])
+    np.testing.assert_array_equal(d.read("Qx"), ref["Qx"])
+    np.testing.assert_array_equal(d.read("Qyold"), ref["Qyold"])
+    np.testing.assert_array_equal(d.dt_log(), ref["dt"])
 
 
 @pytest.mark.parametrize("edges", [(False, False, False, False), (True, False, True, False),
@@ -57,6 +63,25 @@ def test_fp64_edges(gpu, edges):
     assert float(np.max(np.abs(d.read("H") - ref["H"]))) <= FP64_TOL
     assert _faces_close(d.read("Qx"), ref["Qx"], 1e-8)
     assert _faces_close(d.read("Qy"), ref["Qy"], 1e-8)
+    np.testing.assert_array_equal(d.read("H"), ref["H"])
+    np.testing.assert_array_equal(d.read("Qy"), ref["Qy"])
+
+
+def test_fp64_bitwise_on_unstable_flood(gpu):
+    """A flood that the scheme amplifies (HFIX inflow over fractal terrain,
+    depths growing to tens of metres): a 1-ulp difference anywhere grows
+    about tenfold every 50 steps here, so only bitwise agreement keeps the
+    FP64 path on the oracle's trajectory."""
+    scn = generators.valley_case(n=384, n_qvar=20)
+    scn.sim_time = 150.0
+    ref = oracle.run(scn)
+    print(f"valley 384: oracle steps={ref['steps']}", flush=True)
+    d = Domain(scn, precision="fp64", tuning=make_tuning(dt_log_cap=1 << 16))
+    d.run(check_every=72, graph=True)
+    assert d.info()["steps"] == ref["steps"]
+    np.testing.assert_array_equal(d.dt_log(), ref["dt"])
+    np.testing.assert_array_equal(d.read("H"), ref["H"])
+    np.testing.assert_array_equal(d.read("Qx"), ref["Qx"])
 
 
 def test_plane_case_fp64_full_run(gpu):
@@ -156,3 +181,18 @@ def test_stop_and_resume_equals_uninterrupted(gpu, prec):
     for what in ("H", "Qx", "Qy"):
         assert np.array_equal(a.read(what), b.read(what)), what
     assert np.array_equal(a.reports(), b.reports())
+
+
+def test_autotune_picks_a_candidate_and_caches(gpu, tmp_path):
+    from adaptflood_amd.tune import autotune
+    scns = [_small(60 + k, nx, ny, sim_time=50.0) for k, (nx, ny) in
+            enumerate([(300, 40), (90, 130)])]
+    cache = str(tmp_path / "tune.json")
+    tun, table = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), launches=5,
+                          cache_path=cache)
+    assert len(table) == 4
+    assert (tun.tile_rows, tun.variant) in {(r, v) for _, r, v, _ in table}
+    assert table[0][1] == tun.tile_rows and table[0][2] == tun.variant
+    again, table2 = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), cache_path=cache)
+    assert (again.tile_rows, again.variant) == (tun.tile_rows, tun.variant)
+    assert len(table2) == 4

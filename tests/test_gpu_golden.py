@@ -26,6 +26,7 @@ def test_fused_fp64_snapshots_and_end(gpu):
         err = float(np.max(np.abs(d.read("H") - g[f"H_{k}"])))
         print(f"fp64 step {k}: max|dH|={err:.3e}", flush=True)
         assert err <= FP64_TOL
+        np.testing.assert_array_equal(d.read("H"), g[f"H_{k}"])
         assert d.info()["t"] == pytest.approx(float(g[f"t_{k}"]), abs=1e-9)
     assert float(np.max(np.abs(d.read("Qx") - g["Qx_100"]))) <= 1e-8
     assert float(np.max(np.abs(d.read("Qy") - g["Qy_100"]))) <= 1e-8
@@ -36,6 +37,8 @@ def test_fused_fp64_snapshots_and_end(gpu):
     err = float(np.max(np.abs(d.read("H") - g["H_end"])))
     print(f"fp64 end: max|dH|={err:.3e}", flush=True)
     assert err <= FP64_TOL
+    np.testing.assert_array_equal(d.read("H"), g["H_end"])
+    np.testing.assert_array_equal(d.read("Qx"), g["Qx_end"])
     rep = d.reports()
     assert rep.shape == g["reports"].shape
     np.testing.assert_allclose(rep[:, 2], g["reports"][:, 2], rtol=1e-12)
@@ -66,4 +69,6 @@ def test_compat_sequence_end(gpu):
     err = float(np.max(np.abs(r["H"] - g["H_end"])))
     print(f"compat end: max|dH|={err:.3e}", flush=True)
     assert err <= FP64_TOL
+    np.testing.assert_array_equal(r["H"], g["H_end"])
+    np.testing.assert_array_equal(r["Qy"], g["Qy_end"])
     np.testing.assert_allclose(np.array(r["reports"])[:, 2], g["reports"][:, 2], rtol=1e-12)

@@ -369,3 +369,46 @@ def test_velocity_kat():
     # cell (0,1): top -.4/2/2 = -.1, bottom .8/2/2 = .2 -> .2
     # cell (1,1): top .2/2/.25 = .4, bottom -.1/2/.25 = -.2 -> .4
     np.testing.assert_allclose(Vy, [-0.4, 0.3, 0.0, 0.2, 0.4, 0.0], rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------------ pow(h, 10/3) (a2/a3)
+def _ulps(a, b):
+    import struct
+    ia = struct.unpack("<q", struct.pack("<d", a))[0]
+    ib = struct.unpack("<q", struct.pack("<d", b))[0]
+    return abs(ia - ib)
+
+
+def test_pow_10_3_within_one_ulp_of_libm():
+    """The oracle's pow(h, 10.0/3.0) against the C library's pow over the
+    depths the flux sees (and far beyond): never more than 1 ulp apart, and
+    equal in nearly every case."""
+    rng = np.random.default_rng(7)
+    hs = np.exp(rng.uniform(math.log(1e-6), math.log(1e4), 60000))
+    worst, differ = 0, 0
+    for h in hs.tolist():
+        u = _ulps(oracle.pow_10_3(h), math.pow(h, 10.0 / 3.0))
+        worst = max(worst, u)
+        differ += u != 0
+    assert worst <= 1
+    assert differ / len(hs) < 0.01
+
+
+def test_pow_10_3_correctly_rounded_against_decimal():
+    """Against an 80-digit decimal evaluation of h ** (10.0/3.0): correctly
+    rounded in all but a few in 10^4 arguments, never off by more than 1 ulp;
+    exact powers of two come out exactly (8^(10/3) = 1024 * 8^1.48e-16)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 80
+    y = Decimal(10.0 / 3.0)
+    rng = np.random.default_rng(8)
+    hs = np.exp(rng.uniform(math.log(1e-4), math.log(100.0), 4000)).tolist()
+    hs += [1.0, 2.0, 8.0, 0.125, 10.0, 0.001, 1e-3 * (1 + 2 ** -52)]
+    off = 0
+    for h in hs:
+        exact = float(Decimal(h) ** y)
+        u = _ulps(oracle.pow_10_3(h), exact)
+        assert u <= 1, h
+        off += u
+    assert off <= 4
+    assert oracle.pow_10_3(1.0) == 1.0
