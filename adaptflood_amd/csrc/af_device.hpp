@@ -148,6 +148,19 @@ struct FluxConstsF32 {
     float sc4;      // 0.25/dx: sum of four Q -> transverse q_old average
 };
 
+// Surface step and flow depth of a face in FP32 without forming the surface
+// elevations z + h: tens of metres of terrain round to about 1e-6 m in FP32,
+// while the terrain step dz = z0 - z1 and the depths are small numbers.
+//   dh    = (z0 + h0) - (z1 + h1)          = dz + (h0 - h1)
+//   hflow = max(z0+h0, z1+h1) - max(z0, z1) = dz >= 0 ? max(h0, h1 - dz)
+//                                                     : max(h0 + dz, h1)
+__device__ __forceinline__ void face_depths_f32(float z0, float z1, float h0, float h1, float& dh,
+                                                float& hflow) {
+    const float dz = z0 - z1;
+    dh = dz + (h0 - h1);
+    hflow = (dz >= 0.0f) ? getmax(h0, h1 - dz) : getmax(h0 + dz, h1);
+}
+
 // FP32 form of face_flux_ref_t1.  Same physics, rearranged for the VALU:
 //   g*dt*hflow*Sf          = -(g*dt/dx)*hflow*dh
 //   g*dt*hflow*n^2*|q| / hflow^(10/3) = g*dt*n^2*|q| * hflow^(-7/3)
@@ -155,9 +168,8 @@ struct FluxConstsF32 {
 // units (hflow > DepthThresh > 0 here) and one reciprocal for num/den.
 __device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, float h1, float q0,
                                                float qt, const FluxConstsF32& k) {
-    float y0 = z0 + h0, y1 = z1 + h1;
-    float dh = y0 - y1;
-    float hflow = getmax(y0, y1) - getmax(z0, z1);
+    float dh, hflow;
+    face_depths_f32(z0, z1, h0, h1, dh, hflow);
     hflow = getmax(hflow, 0.0f);
     hflow = getmin(hflow, k.maxh);
     float Q = 0.0f;
@@ -175,9 +187,8 @@ __device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, flo
 __device__ __forceinline__ float face_flux_f32_q(float z0, float z1, float h0, float h1, float Q0,
                                                  float a, float b, float c, float d,
                                                  const FluxConstsF32& k) {
-    float y0 = z0 + h0, y1 = z1 + h1;
-    float dh = y0 - y1;
-    float hflow = getmax(y0, y1) - getmax(z0, z1);
+    float dh, hflow;
+    face_depths_f32(z0, z1, h0, h1, dh, hflow);
     hflow = getmax(hflow, 0.0f);
     hflow = getmin(hflow, k.maxh);
     float Q = 0.0f;
