@@ -29,7 +29,7 @@ def _faces_close(a, b, tol):
 
 @pytest.mark.parametrize("seed,nx,ny,rows", [
     (1, 37, 29, 8), (2, 64, 64, 16), (3, 130, 70, 32), (4, 257, 19, 5), (5, 300, 300, 128),
-    (6, 1100, 40, 16),
+    (6, 1100, 40, 16), (7, 256, 48, 8), (8, 512, 20, 7),
 ])
 def test_fp64_matches_oracle_small(gpu, seed, nx, ny, rows):
     scn = _small(seed, nx, ny, sim_time=200.0)
@@ -104,6 +104,20 @@ def test_plane_case_fp64_full_run(gpu):
     np.testing.assert_allclose(rep[:, 3], ref["reports"][:, 3], rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("nx", [256, 512, 513])
+def test_fp32_strip_multiple_widths(gpu, nx):
+    """Grids whose width is a whole number of FP32 strips (256 columns), and
+    one column more: the east boundary face sits just outside the last strip
+    (read through the strip-edge halo) or inside it."""
+    scn = _small(70 + nx % 7, nx, 40, sim_time=120.0)
+    ref = oracle.run(scn)
+    d = Domain(scn, precision="fp32", tuning=make_tuning(tile_rows=8))
+    d.run(check_every=30, graph=False)
+    err = float(np.max(np.abs(d.read("H") - ref["H"])))
+    print(f"fp32 {nx}x40: max|dH|={err:.2e}", flush=True)
+    assert err <= FP32_TOL
+
+
 def test_plane_case_fp32_full_run(gpu):
     scn = generators.plane_case()
     ref = oracle.run(scn)
@@ -137,7 +151,7 @@ def test_tiling_invariance(gpu, prec):
     a wide grid (nx = 2100) takes the 1 KiB row pitch."""
     scn = _small(31, 2100, 141, sim_time=60.0)
     outs = []
-    for rows, wpb, var in ((4, 1, 1), (32, 4, 0), (128, 4, 2), (61, 4, 3)):
+    for rows, wpb, var in ((4, 1, 1), (32, 4, 0), (128, 4, 2), (61, 4, 3), (33, 1, 2)):
         d = Domain(scn, precision=prec,
                    tuning=make_tuning(tile_rows=rows, waves_per_block=wpb, variant=var))
         d.run(check_every=25, graph=False)
