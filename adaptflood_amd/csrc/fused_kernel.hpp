@@ -358,10 +358,13 @@ struct Stepper {
                 ox[v] = (!EDGE || c0 + v <= nx) ? o.qx[v] : T(0);
                 oy[v] = (!EDGE || c0 + v < nx) ? o.qy[v] : T(0);
             }
+            // streaming stores: the new state is not read again in this
+            // launch, so it is written with the non-temporal hint (measured
+            // +4% at FP32, +2% at FP64 on the 16384 x 16384 step)
             const long long rowo = (long long)j * x.P + c0;
-            *(gptr<Vec<T, V>>)(x.Hn + rowo) = oh;
-            *(gptr<Vec<T, V>>)(x.Qxn + rowo) = ox;
-            *(gptr<Vec<T, V>>)(x.Qyn + rowo + x.P) = oy;
+            __builtin_nontemporal_store(oh, (gptr<Vec<T, V>>)(x.Hn + rowo));
+            __builtin_nontemporal_store(ox, (gptr<Vec<T, V>>)(x.Qxn + rowo));
+            __builtin_nontemporal_store(oy, (gptr<Vec<T, V>>)(x.Qyn + rowo + x.P));
         }
         if (EDGE && x.lane == 63 && x.i0 + W == nx) x.Qxn[(long long)j * x.P + nx] = o.qxe;
     }
