@@ -379,6 +379,54 @@ def bench_accuracy(args):
                       "results": lines}), flush=True)
 
 
+def bench_divergence(args):
+    """How far two runs of the same case drift apart at equal step counts:
+    the FP64 fused path against the CPU oracle, and FP32 against FP64, on the
+    4096^2 valley case (an inflow the scheme amplifies) and an 8192^2 roofline
+    case.  Prints max |dH| every `every` steps."""
+    import torch
+    from adaptflood_amd import generators
+    from adaptflood_amd.domain import Domain, make_tuning
+    from oracle import oracle
+    out = {"metric": "max |dH| between runs at equal step counts", "series": []}
+
+    def record(name, rows):
+        out["series"].append({"case": name, "points": rows})
+        print(f"[div] {name}: " + ", ".join(f"{k}:{v:.2e}" for k, v, _ in rows),
+              file=sys.stderr, flush=True)
+
+    valley = generators.valley_case(n=args.valley_size)
+    r = oracle.OracleRun(valley, dt_log_cap=1, mass_log_cap=1)
+    d = Domain(valley, precision="fp64", tuning=make_tuning())
+    rows, done = [], 0
+    while done < 600:
+        r.advance(100)
+        d.advance(100)
+        done += 100
+        diff = float(np.max(np.abs(d.read("H").ravel() - r.H)))
+        rows.append((done, diff, None))
+    record(f"valley {valley.nx}^2: FP64 fused vs FP64 CPU oracle", rows)
+    d.close()
+    del r
+    cases = [("valley", valley, 400, 40), ("roofline", generators.roofline_case(n=8192), 1200, 200)]
+    for name, scn, steps, every in cases:
+        a = Domain(scn, precision="fp64", tuning=make_tuning())
+        b = Domain(scn, precision="fp32", tuning=make_tuning())
+        rows, done = [], 0
+        while done < steps:
+            a.advance(every)
+            b.advance(every)
+            done += every
+            Ha, Hb = a.read("H"), b.read("H")
+            rms = float(np.sqrt(np.mean((Ha - Hb) ** 2)))
+            rows.append((done, float(np.max(np.abs(Ha - Hb))), rms))
+        record(f"{name} {scn.nx}^2: FP32 vs FP64 fused", rows)
+        a.close()
+        b.close()
+    torch.cuda.synchronize()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,7 +443,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 4 GiB device-to-device copy (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
-    ap.add_argument("--workload", choices=("roofline", "ensemble", "served", "accuracy"),
+    ap.add_argument("--workload",
+                    choices=("roofline", "ensemble", "served", "accuracy", "divergence"),
                     default="roofline")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
     ap.add_argument("--acc-time", type=float, default=600.0,
@@ -426,6 +475,8 @@ def main():
         return bench_served(args)
     if args.workload == "accuracy":
         return bench_accuracy(args)
+    if args.workload == "divergence":
+        return bench_divergence(args)
     import torch
     from adaptflood_amd import generators
     if not torch.cuda.is_available():
