@@ -3,9 +3,15 @@
 Default workload (one MI355X): the 16384 x 16384 roofline case at 2 m cells,
 fractal DEM with 10% NODATA blobs, 5000 QFIX sources, all edges FREE, FP32
 state, starting dry; W untimed warm-up steps, then K timed steps.  Prints one
-JSON line.  The same K steps are also measured at FP64 (key "fp64"), and the
-FP64 CPU oracle is timed on a bounded sample of the same workload
-("cpu_baseline").
+JSON line.  The same K steps are also measured at FP64 (key "fp64"); the
+served ensemble (512 requests, Poisson arrivals at 20/s, continuous batching)
+reports its p50/p99 completion latency (key "served"); and the FP64 CPU
+oracle is timed on a bounded sample of the roofline workload ("cpu_baseline").
+
+Other workloads (--workload): "ensemble" (512 ragged scenarios in one launch
+per step against sequential single-scenario runs), "served" (the served
+ensemble with its sequential baselines), "accuracy" (FP32/FP64 errors against
+FP64 references) and "divergence" (lockstep drift at equal step counts).
 
 Algorithmic bytes per cell-update: read z, H, Qx_old, Qy_old and write H, Qx,
 Qy = 7 values = 28 B at FP32 (56 B at FP64).
@@ -210,6 +216,30 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_served(scns, arrivals, prec, budget_gb, chunk):
+    """The continuous-batching loop over requests arriving at `arrivals` (s)."""
+    import torch
+    from adaptflood_amd import generators
+    from adaptflood_amd.serve import Pool, Request, serve
+    n = len(scns)
+    free, _ = torch.cuda.mem_get_info()
+    budget = min(int(budget_gb * (1 << 30)), int(free * 0.8))
+    pool = Pool(budget_bytes=budget, precision=prec, max_scenarios=n, max_tiles=1 << 20)
+    # a few requests through a warm pool first, so kernel loading is not timed
+    warm = [Request(-1 - k, generators.ensemble_member(k, seed_base=5000, sim_time=400.0), 0.0)
+            for k in range(2)]
+    serve(warm, pool, chunk_steps=36, clock="wall")
+    reqs = [Request(k, s, float(a)) for k, (s, a) in enumerate(zip(scns, arrivals))]
+    done, summary = serve(reqs, pool, chunk_steps=chunk, clock="wall")
+    print(f"[bench] served {summary['requests']} in {summary['wall_s']:.1f}s: "
+          f"{summary['gcups']:.1f} GCUPS, p50 {summary['p50_latency']:.2f}s, "
+          f"p99 {summary['p99_latency']:.2f}s", file=sys.stderr, flush=True)
+    pool.close()
+    del pool
+    torch.cuda.empty_cache()
+    return done, summary, budget
+
+
 def bench_served(args):
     """Poisson arrivals (20/s) of ensemble scenarios with sim_time U(600, 3600) s,
     admitted into a pool by HBM footprint, advanced together, released when done."""
@@ -222,26 +252,11 @@ def bench_served(args):
     print(f"[bench] generated {n} requests, {cells} cells, in {time.perf_counter() - t:.1f}s",
           file=sys.stderr, flush=True)
     import torch
-    from adaptflood_amd.serve import Pool, Request, serve
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     prec = args.precision
-    free, _ = torch.cuda.mem_get_info()
-    budget = min(int(args.budget_gb * (1 << 30)), int(free * 0.8))
-    pool = Pool(budget_bytes=budget, precision=prec, max_scenarios=n, max_tiles=1 << 20)
-    # a few requests through a warm pool first, so kernel loading is not timed
-    warm = [Request(-1 - k, generators.ensemble_member(k, seed_base=5000, sim_time=400.0), 0.0)
-            for k in range(2)]
-    serve(warm, pool, chunk_steps=36, clock="wall")
-    reqs = [Request(k, s, float(a)) for k, (s, a) in enumerate(zip(scns, arr))]
-    done, summary = serve(reqs, pool, chunk_steps=args.chunk, clock="wall")
-    print(f"[bench] served {summary['requests']} in {summary['wall_s']:.1f}s: "
-          f"{summary['gcups']:.1f} GCUPS, p50 {summary['p50_latency']:.2f}s, "
-          f"p99 {summary['p99_latency']:.2f}s", file=sys.stderr, flush=True)
-    pool.close()
-    del pool
-    torch.cuda.empty_cache()
+    done, summary, budget = run_served(scns, arr, prec, args.budget_gb, args.chunk)
     sample = scns[::args.seq_stride]
     g_fused, s_fused = sequential_fused(sample, prec, args.seq_steps)
     print(f"[bench] sequential fused {prec} ({len(sample)} scenarios): {g_fused:.2f} GCUPS",
@@ -438,8 +453,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-fp64", action="store_true",
                     help="measure only the FP64 step (for profiling runs)")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=80,
+                    help="CPU oracle steps of the baseline sample (about 15 s on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-served", action="store_true",
+                    help="skip the served-ensemble latency run of the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 4 GiB device-to-device copy (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -477,6 +495,16 @@ def main():
         return bench_accuracy(args)
     if args.workload == "divergence":
         return bench_divergence(args)
+    served_reqs = None
+    if not args.no_served:
+        # generated in worker processes before this process touches the GPU
+        from adaptflood_amd import generators as gen
+        arr, sims = gen.arrivals(args.members, rate=args.rate, seed=5)
+        t = time.perf_counter()
+        reqs = generate_members(list(range(args.members)), 5000, [float(x) for x in sims])
+        served_reqs = (reqs, arr)
+        print(f"[bench] generated {len(reqs)} served requests in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
     import torch
     from adaptflood_amd import generators
     if not torch.cuda.is_available():
@@ -544,6 +572,17 @@ def main():
                                     "frac": round(a64 / HBM_PEAK_GBS, 4),
                                     "traffic": _traffic_from_profile("fp64"),
                                     "kernel_ms": round(kern64 * 1e3, 4)}}
+    if served_reqs is not None:
+        reqs, arr = served_reqs
+        done, summary, budget = run_served(reqs, arr, "fp32", args.budget_gb, args.chunk)
+        out["served"] = {
+            "workload": f"{len(reqs)} requests from the ensemble generator (seed 5), Poisson "
+                        f"arrivals at {args.rate}/s, sim_time U(600,3600) s, FP32, HBM budget "
+                        f"{budget / (1 << 30):.0f} GiB, continuous batching",
+            "p50_latency_s": round(summary["p50_latency"], 3),
+            "p99_latency_s": round(summary["p99_latency"], 3),
+            "gcups": round(summary["gcups"], 3), "wall_s": round(summary["wall_s"], 3),
+            "arrival_span_s": round(float(arr[-1]), 3)}
     if not args.no_cpu_baseline:
         from oracle import oracle
         r = oracle.OracleRun(scn, dt_log_cap=args.cpu_steps + 1, mass_log_cap=4)

@@ -27,3 +27,12 @@ def test_missing_par_file(exe, tmp_path):
     r = subprocess.run([exe, str(tmp_path / "none.par")], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "lisflood_amd: load" in r.stderr
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before any
+    GPU work."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["python", os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
