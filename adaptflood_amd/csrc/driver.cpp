@@ -13,7 +13,8 @@
 // Console lines and output files follow main.cpp: the mass report line
 // "t=%.0fs: Tstep=%.2f, Vol=%.0fm3, Area=%.0fm2" (main.cpp:300), H_%04d.asc
 // at t = 0 and at every save, H_final.asc, Vx_final.asc / Vy_final.asc when
-// the .par file asks for velocity output (main.cpp:380-392).
+// the .par file asks for velocity output (main.cpp:380-392), and stage.csv
+// when it names a stage file (main.cpp:95-113, 306-322).
 #include <hip/hip_runtime_api.h>
 #include <sys/stat.h>
 
@@ -55,10 +56,51 @@ struct Grid {
     double xll = 0, yll = 0, cell = 0, nodata = 0;
 };
 
+// Stage gauges and their CSV (main.cpp:95-113, 306-322): one line per step
+// that ends at t >= next_stage, "%.1f" time then ",%.4f" per gauge (H at the
+// gauge cell) or ",NaN" when the cell index falls outside the grid.
+struct Stages {
+    FILE* fp = nullptr;
+    std::vector<int64_t> idx;
+    double interval = 0.0;
+    void open(af_case* c, const af_par& par, int nx) {
+        int32_t n = 0;
+        af_ok(af_case_stage(c, -1, nullptr, nullptr, nullptr, 0, &n), "stages");
+        if (!par.stage_file[0] || n == 0) return;
+        const std::string path = std::string(par.output_dir) + "/stage.csv";
+        fp = fopen(path.c_str(), "w");
+        if (!fp) return;
+        fprintf(fp, "Time(s)");
+        for (int32_t i = 0; i < n; i++) {
+            int32_t gx = 0, gy = 0;
+            char name[256];
+            af_ok(af_case_stage(c, i, &gx, &gy, name, sizeof name, &n), "stage");
+            fprintf(fp, ",%s", name);
+            idx.push_back((int64_t)gx + (int64_t)gy * nx);
+        }
+        fprintf(fp, "\n");
+        interval = par.stage_int;
+    }
+    // writes one line from the gauge values
+    void line(double t, const std::vector<double>& v) {
+        fprintf(fp, "%.1f", t);
+        for (double x : v) {
+            if (x != x) fprintf(fp, ",NaN");
+            else fprintf(fp, ",%.4f", x);
+        }
+        fprintf(fp, "\n");
+        fflush(fp);
+    }
+    ~Stages() {
+        if (fp) fclose(fp);
+    }
+};
+
 struct Output {
     std::string dir;
     Grid g;
     bool quiet = false;
+    Stages* stages = nullptr;
     void asc(const char* name, const std::vector<double>& data, const char* note = "") const {
         const std::string path = dir + "/" + name;
         af_ok(af_write_asc(path.c_str(), data.data(), g.nx, g.ny, g.xll, g.yll, g.cell),
@@ -172,6 +214,7 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
     out.asc(name, Hh, " (initial state)");
 
     double t = 0.0, next_save = par.save_int, next_mass = par.mass_int;
+    double next_stage = par.stage_int;
     long long step = 0;
     while (t < d.sim_time) {
         double Tstep = d.init_tstep;
@@ -210,6 +253,18 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
             mass_of(Hh, dA, &vol, &area);
             report_line(t, Tstep, vol, area);
             next_mass += d.mass_int;
+        }
+        if (out.stages && out.stages->fp && t >= next_stage) {
+            std::vector<double> v(out.stages->idx.size());
+            const long long ncell = (long long)nx * ny;
+            for (size_t g = 0; g < v.size(); g++) {
+                const int64_t k = out.stages->idx[g];
+                v[g] = NAN;
+                if (k >= 0 && k < ncell)
+                    hip_ok(hipMemcpy(&v[g], H + k, 8, hipMemcpyDeviceToHost), "gauge");
+            }
+            out.stages->line(t, v);
+            next_stage += out.stages->interval;
         }
         if (t >= next_save) {
             pull_H();
@@ -269,11 +324,15 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     af_run_info info;
     af_ok(af_domain_info(D, &info, nullptr), "info");
     double next_save = par.save_int;
+    const bool gauges = out.stages && out.stages->fp;
+    double next_stage = par.stage_int;
     while (info.t < d.sim_time) {
-        const double stop = (par.save_int > 0 && next_save < d.sim_time) ? next_save : d.sim_time;
+        double stop = d.sim_time;
+        if (par.save_int > 0 && next_save < stop) stop = next_save;
+        if (gauges && next_stage < stop) stop = next_stage;  // interval 0: every step
         if (info.t >= stop) {
-            // one step jumped past two save times: the reference saves again
-            // after the next step
+            // one step jumped past two output times: the reference writes
+            // again after the next step
             af_ok(af_batch_set_sim_time(D, 0, d.sim_time, nullptr), "set_sim_time");
             af_ok(af_domain_advance(D, 1, 0, nullptr), "advance");
         } else {
@@ -282,6 +341,14 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
         }
         af_ok(af_domain_info(D, &info, nullptr), "info");
         show_reports();
+        if (gauges && info.t >= next_stage) {
+            std::vector<double> v(out.stages->idx.size());
+            af_ok(af_batch_read_cells(D, 0, out.stages->idx.data(), (int)v.size(), v.data(),
+                                      nullptr),
+                  "gauges");
+            out.stages->line(info.t, v);
+            next_stage += out.stages->interval;
+        }
         if (info.t >= next_save) {
             af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
             snprintf(name, sizeof name, "H_%04d.asc", save_num++);
@@ -339,6 +406,9 @@ int main(int argc, char** argv) {
         printf("Grid: %d x %d, cellsize %.3f, sim time %.0f s, save every %.0f s\n", d.nx, d.ny,
                d.dx, d.sim_time, par.save_int);
         mkdir(par.output_dir, 0755);
+        Stages stages;
+        stages.open(c, par, d.nx);
+        out.stages = &stages;
         const auto w0 = std::chrono::steady_clock::now();
         Result r = path == "compat" ? run_compat(d, par, out)
                                     : run_fused(d, par, out, path == "fp64" ? AF_FP64 : AF_FP32);

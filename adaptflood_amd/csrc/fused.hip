@@ -1035,6 +1035,37 @@ int af_batch_read(af_domain* D, int s, int which, double* out, af_stream_t strea
     return AF_OK;
 }
 
+int af_batch_read_cells(af_domain* D, int s, const int64_t* idx, int n, double* out,
+                        af_stream_t stream) {
+    AF_REQUIRE(D && (n == 0 || (idx && out)) && n >= 0 && s >= 0 && s < D->max_scn &&
+                   D->scn[s].active,
+               AF_E_INVALID, "read_cells: bad arguments");
+    hipStream_t st = to_stream(stream);
+    HostScn& h = D->scn[s];
+    const int par = current_parity(D, s, nullptr, st);
+    if (par < 0) return par;
+    const size_t es = elem_size(D->precision);
+    const long long ncell = (long long)h.nx * h.ny;
+    std::vector<char> v((size_t)n * es);
+    for (int k = 0; k < n; k++) {
+        if (idx[k] < 0 || idx[k] >= ncell) continue;
+        const long long j = idx[k] / h.nx, i = idx[k] % h.nx;
+        AF_HIP(hipMemcpyAsync(v.data() + (size_t)k * es,
+                              (const char*)h.dev.H[par] + (size_t)(j * h.P + i) * es, es,
+                              hipMemcpyDeviceToHost, st));
+    }
+    AF_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < n; k++) {
+        if (idx[k] < 0 || idx[k] >= ncell) {
+            out[k] = NAN;
+            continue;
+        }
+        out[k] = es == 8 ? reinterpret_cast<const double*>(v.data())[k]
+                         : (double)reinterpret_cast<const float*>(v.data())[k];
+    }
+    return AF_OK;
+}
+
 int af_domain_read(af_domain* D, int which, double* out, af_stream_t stream) {
     return af_batch_read(D, 0, which, out, stream);
 }

@@ -49,6 +49,9 @@ struct af_case {
     std::vector<char> line_sides;  // FREE sides in file order
     bool rain_enabled = false;
     std::vector<double> rain_t, rain_r;
+    // stage gauges: grid coordinates and names, in file order
+    std::vector<int32_t> gx, gy;
+    std::vector<std::string> gname;
     // flattened views for af_case_scenario
     std::vector<int32_t> px, py, ptype, poff, plen;
     std::vector<double> pvalue, st, sv;
@@ -389,6 +392,63 @@ int af_case_read_rain(af_case* c, const char* path) {
 }
 
 // The input sequence of main.cpp:24-92.
+// read_stage_file (src/io.cpp:50-85): "x y [name]" per line, empty and
+// '#' lines skipped, world coordinates truncated to grid cells from the top
+// left corner, unnamed gauges called Stage<k>.  A file with no gauge is an
+// error (the reference disables stage output then).
+int af_case_read_stage(af_case* c, const char* path) {
+    if (!c || !path) {
+        af::set_error("read_stage: null argument");
+        return AF_E_INVALID;
+    }
+    if (!c->have_dem) {
+        af::set_error("read_stage: the DEM header is needed first");
+        return AF_E_STATE;
+    }
+    std::ifstream f(path);
+    if (!f.is_open()) {
+        af::set_error("Cannot open stage file %s", path);
+        return AF_E_IO;
+    }
+    c->gx.clear();
+    c->gy.clear();
+    c->gname.clear();
+    const double tly = c->yll + c->nrows * c->cellsize;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        double x, y;
+        if (!(iss >> x >> y)) continue;
+        std::string name;
+        iss >> name;
+        if (name.empty()) name = "Stage" + std::to_string(c->gname.size() + 1);
+        c->gx.push_back((int32_t)((x - c->xll) / c->cellsize));
+        c->gy.push_back((int32_t)((tly - y) / c->cellsize));
+        c->gname.push_back(name);
+    }
+    if (c->gname.empty()) {
+        af::set_error("no stage points in %s", path);
+        return AF_E_IO;
+    }
+    return AF_OK;
+}
+
+int af_case_stage(af_case* c, int32_t i, int32_t* grid_x, int32_t* grid_y, char* name,
+                  int32_t cap, int32_t* n) {
+    if (!c) return AF_E_INVALID;
+    if (n) *n = (int32_t)c->gname.size();
+    if (i < 0) return AF_OK;
+    if (i >= (int32_t)c->gname.size()) {
+        af::set_error("stage %d of %zu", i, c->gname.size());
+        return AF_E_INVALID;
+    }
+    if (grid_x) *grid_x = c->gx[i];
+    if (grid_y) *grid_y = c->gy[i];
+    if (name && cap > 0) copy_str(name, (size_t)cap, c->gname[i]);
+    return AF_OK;
+}
+
 int af_case_load(af_case* c, const char* par_path) {
     if (!c || !par_path) {
         af::set_error("case_load: null argument");
@@ -400,6 +460,12 @@ int af_case_load(af_case* c, const char* par_path) {
     if ((rc = af_case_read_bci(c, c->par.bci_file))) return rc;
     if ((rc = af_case_read_bdy(c, c->par.bdy_file))) return rc;
     if (c->par.rain_file[0]) (void)af_case_read_rain(c, c->par.rain_file);  // optional
+    // optional, as main.cpp:95-113: a missing or empty file only disables the output
+    if (c->par.stage_file[0] && af_case_read_stage(c, c->par.stage_file) != AF_OK) {
+        c->gx.clear();
+        c->gy.clear();
+        c->gname.clear();
+    }
     return AF_OK;
 }
 

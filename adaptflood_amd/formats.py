@@ -116,6 +116,21 @@ class Case:
              C.byref(n))
         return bool(en.value), t[: n.value], r[: n.value]
 
+    def read_stage(self, path):
+        call("af_case_read_stage", self.h, path.encode())
+
+    def stages(self):
+        """Stage gauges as (name, grid_x, grid_y) in file order."""
+        n = C.c_int32()
+        call("af_case_stage", self.h, -1, None, None, None, 0, C.byref(n))
+        out = []
+        for i in range(n.value):
+            gx, gy = C.c_int32(), C.c_int32()
+            name = C.create_string_buffer(256)
+            call("af_case_stage", self.h, i, C.byref(gx), C.byref(gy), name, 256, C.byref(n))
+            out.append((name.value.decode(), gx.value, gy.value))
+        return out
+
     def scenario(self):
         """The Scenario main.cpp would step (constants of main.cpp:115-127)."""
         d = _lib.ScenarioDesc()

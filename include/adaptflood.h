@@ -232,6 +232,10 @@ int af_batch_info(af_domain* d, int scenario, af_run_info* info, af_stream_t str
  * output and moves the stop time on reproduces the reference's save cadence
  * (main.cpp:327): the run stops at the first step with t >= the stop time. */
 int af_batch_set_sim_time(af_domain* d, int scenario, double sim_time, af_stream_t stream);
+/* H at n cells given as reference indices i + j*nx (the stage gauges of
+ * main.cpp:306-322); indices outside the grid give NaN.  `out` is host memory. */
+int af_batch_read_cells(af_domain* d, int scenario, const int64_t* idx, int n, double* out,
+                        af_stream_t stream);
 int af_batch_read(af_domain* d, int scenario, int which, double* out, af_stream_t stream);
 int af_batch_read_dt_log(af_domain* d, int scenario, double* out, int64_t cap, int64_t* n,
                          af_stream_t stream);
@@ -290,8 +294,14 @@ int af_case_read_bci(af_case* c, const char* path);
 int af_case_read_bdy(af_case* c, const char* path);
 /* Replaces read_rain (src/io.cpp:103-129). */
 int af_case_read_rain(af_case* c, const char* path);
-/* Runs the input sequence of main.cpp:24-92 for a .par file: par, DEM, BCI,
- * BDY (all required) and the optional rain file. */
+/* Replaces read_stage_file (src/io.cpp:50-85); needs the DEM header first. */
+int af_case_read_stage(af_case* c, const char* path);
+/* Stage gauge i: grid x, y and name (cap bytes); *n = number of gauges
+ * (i < 0 only counts). */
+int af_case_stage(af_case* c, int32_t i, int32_t* grid_x, int32_t* grid_y, char* name,
+                  int32_t cap, int32_t* n);
+/* Runs the input sequence of main.cpp:24-113 for a .par file: par, DEM, BCI,
+ * BDY (all required), the optional rain file and the optional stage file. */
 int af_case_load(af_case* c, const char* par_path);
 /* Fills `desc` with pointers into the case (valid while the case lives),
  * with the constants of main.cpp:115-127. */

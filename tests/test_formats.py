@@ -164,6 +164,27 @@ def test_rain_file(tmp_path):
     assert c2.rain()[0] is False
 
 
+def test_stage_file_world_coordinates_and_names(tmp_path):
+    """read_stage_file (io.cpp:50-85): '#' and empty lines skipped, lines that
+    do not start with two numbers skipped, unnamed gauges numbered by count,
+    world -> grid by truncation from the top-left corner."""
+    c = _case_with_dem(tmp_path)
+    # tly = 2080; gx = int((x-1000)/10); gy = int((2080-y)/10)
+    p = _w(tmp_path, "s.txt", "\n".join([
+        "# gauges",
+        "1015 2075 bridge",   # (1, 0)
+        "",
+        "1099.9 2000.1",      # (9, 7), Stage2
+        "not a gauge",
+        "1200 2075 far",      # (20, 0): outside the grid, kept
+    ]) + "\n")
+    c.read_stage(p)
+    assert c.stages() == [("bridge", 1, 0), ("Stage2", 9, 7), ("far", 20, 0)]
+    empty = _w(tmp_path, "e.txt", "# nothing\n")
+    with pytest.raises(Exception):
+        c.read_stage(empty)
+
+
 def test_interp_matches_reference_rules():
     from adaptflood_amd.kernels import interpolate_bc_value as f
     t = np.array([0.0, 10.0, 10.0, 30.0])

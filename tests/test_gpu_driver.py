@@ -37,13 +37,38 @@ def _save_count(dts, save_int, sim_time):
     return n
 
 
+def _oracle_stages(scn, cells, interval):
+    """Gauge lines the reference writes: after every step that ends at
+    t >= next_stage (main.cpp:306-322)."""
+    r = oracle.OracleRun(scn)
+    nxt, lines = interval, []
+    while r.t < scn.sim_time:
+        r.advance(1)
+        if r.t >= nxt:
+            lines.append([f"{r.t:.1f}"] + [f"{r.H[k]:.4f}" if 0 <= k < r.H.size else "NaN"
+                                           for k in cells])
+            nxt += interval
+    return lines
+
+
 @pytest.mark.parametrize("path,tol", [("compat", 1e-4), ("fp64", 1e-4), ("fp32", 1.1e-3)])
 def test_driver_paths(gpu, tmp_path, path, tol):
     scn = generators.small_case(48, 36, seed=17, sim_time=300.0)
     scn.mass_int = 30.0
     for k, p in enumerate(scn.points):
         p.name = f"pt{k}"
-    par = formats.write_case(str(tmp_path / "case"), scn, extra_par="saveint 100\nvoutput\n")
+    case = tmp_path / "case"
+    case.mkdir()
+    # gauges at cells (5, 3) and (40, 30), one east of the grid at (60, 1),
+    # whose flat index still falls inside (main.cpp's only range test)
+    stage = case / "gauges.txt"
+    tly = 36 * scn.dx
+    stage.write_text("# x y name\n"
+                     f"{5.5 * scn.dx} {tly - 3.5 * scn.dx} g1\n"
+                     f"{40.5 * scn.dx} {tly - 30.5 * scn.dx}\n"
+                     f"{60.5 * scn.dx} {tly - 1.5 * scn.dx} east\n")
+    par = formats.write_case(str(case), scn, extra_par=f"saveint 100\nvoutput\n"
+                                                     f"stagefile {stage}\nstageint 50\n")
     back = formats.load_case(par).scenario()
     ref = oracle.run(back)
     print(f"driver --path {path}: oracle steps={ref['steps']}", flush=True)
@@ -61,8 +86,16 @@ def test_driver_paths(gpu, tmp_path, path, tol):
     reps = [REPORT.match(line) for line in r.stdout.splitlines()]
     reps = [m.groups() for m in reps if m]
     assert len(reps) == len(ref["reports"])
+    with open(os.path.join(out, "stage.csv")) as f:
+        rows = [line.strip().split(",") for line in f]
+    assert rows[0] == ["Time(s)", "g1", "Stage2", "east"]
+    want = _oracle_stages(back, [5 + 3 * 48, 40 + 30 * 48, 60 + 1 * 48], 50.0)
+    assert len(rows) - 1 == len(want)
     if path == "fp32":
+        for got, exp in zip(rows[1:], want):
+            assert all(abs(float(a) - float(b)) <= tol for a, b in zip(got[1:], exp[1:]))
         return  # FP32 dt may differ in the last bits; the step count is not pinned
+    assert rows[1:] == want
     assert n_saved == _save_count(ref["dt"], 100.0, 300.0)
     steps = int(re.search(r"Steps: (\d+)", r.stdout).group(1))
     assert steps == ref["steps"]
