@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-pthread"]
 
 HIP_SOURCES = ["compat.hip", "fused.hip"]
 CXX_SOURCES = ["io.cpp", "common.cpp"]
@@ -56,7 +56,7 @@ def build(verbose=False):
         objs.append(obj)
     lib = os.path.join(OUT, "libadaptflood_amd.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"])
     os.replace(tmp, lib)
     drv_src = os.path.join(CSRC, "driver.cpp")
     if os.path.exists(drv_src):

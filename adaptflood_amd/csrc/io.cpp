@@ -18,7 +18,11 @@
 
 #include <fstream>
 #include <sstream>
+#include <algorithm>
+#include <cmath>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/adaptflood.h"
@@ -185,6 +189,107 @@ void af_case_free(af_case* c) { delete c; }
 // read_dem: six positional header pairs (names ignored), then nrows rows of
 // ncols values, north row first.  Values that fail to parse stay 0, as
 // fscanf leaves the zero-initialised vector untouched.
+namespace {
+
+inline bool is_ws(char ch) {
+    return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\v' || ch == '\f';
+}
+
+// The value loop of read_dem (io.cpp:150-155): fscanf("%lf") per cell, in
+// order, until a token does not convert (every later fscanf fails on that
+// same token, so the remaining cells keep 0).  The text is cut at
+// whitespace into one piece per thread; each piece is parsed with strtod
+// (the conversion fscanf uses) and the pieces are concatenated in order up
+// to the first piece that stopped on a bad token.
+void parse_values(const char* beg, const char* end, std::vector<double>& out) {
+    const size_t n = out.size();
+    const size_t bytes = (size_t)(end - beg);
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if (bytes < (1u << 20)) nt = 1;
+    std::vector<const char*> cut(nt + 1);
+    cut[0] = beg;
+    cut[nt] = end;
+    for (unsigned t = 1; t < nt; t++) {
+        const char* p = beg + bytes * t / nt;
+        if (p < cut[t - 1]) p = cut[t - 1];
+        while (p < end && !is_ws(*p)) p++;
+        cut[t] = p;
+    }
+    std::vector<std::vector<double>> part(nt);
+    std::vector<char> stopped(nt, 0);
+    auto work = [&](unsigned t) {
+        const char* q = cut[t];
+        const char* e = cut[t + 1];
+        std::vector<double>& v = part[t];
+        v.reserve((size_t)(e - q) / 6 + 1);
+        while (true) {
+            while (q < e && is_ws(*q)) q++;
+            if (q >= e) break;
+            char* after = nullptr;
+            const double x = strtod(q, &after);
+            if (after == q) {
+                stopped[t] = 1;
+                break;
+            }
+            v.push_back(x);
+            q = after;
+        }
+    };
+    if (nt == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; t++) th.emplace_back(work, t);
+        for (auto& x : th) x.join();
+    }
+    size_t k = 0;
+    for (unsigned t = 0; t < nt && k < n; t++) {
+        const size_t m = std::min(part[t].size(), n - k);
+        std::copy(part[t].begin(), part[t].begin() + m, out.begin() + k);
+        k += m;
+        if (stopped[t]) break;
+    }
+}
+
+// "%.4f " of x, as printf writes it.  printf rounds the exact binary value
+// to 4 decimals.  For |x| < 1e5, y = |x| * 1e4 < 2^30 is the exact product
+// to within half an ulp of y, at most 6e-8; so whenever y is more than 1e-6
+// away from a half-integer, the nearest integer of y is printf's last digit
+// and the digits are formed here directly.  Near-halves, larger values,
+// infinities and NaN go to snprintf.
+int fmt4(char* out, size_t cap, double x) {
+    const double ax = fabs(x);
+    if (!(ax < 1e5)) return snprintf(out, cap, "%.4f ", x);
+    const double y = ax * 10000.0;
+    const double fl = floor(y);
+    const double fr = y - fl;
+    if (fabs(fr - 0.5) < 1e-6) return snprintf(out, cap, "%.4f ", x);
+    const unsigned long long q = (unsigned long long)fl + (fr > 0.5 ? 1 : 0);
+    char tmp[32];
+    int n = 0;
+    unsigned long long ip = q / 10000, fp = q % 10000;
+    // fraction digits (reversed), then the point, then the integer digits
+    for (int d = 0; d < 4; d++) {
+        tmp[n++] = (char)('0' + fp % 10);
+        fp /= 10;
+    }
+    tmp[n++] = '.';
+    do {
+        tmp[n++] = (char)('0' + ip % 10);
+        ip /= 10;
+    } while (ip);
+    int k = 0;
+    if (std::signbit(x)) out[k++] = '-';  // printf keeps the sign of -0.0 and of tiny negatives
+    while (n) out[k++] = tmp[--n];
+    out[k++] = ' ';
+    out[k] = 0;
+    return k;
+}
+
+}  // namespace
+
 int af_case_read_dem(af_case* c, const char* path) {
     if (!c || !path) {
         af::set_error("read_dem: null argument");
@@ -196,10 +301,13 @@ int af_case_read_dem(af_case* c, const char* path) {
         return AF_E_IO;
     }
     std::vector<char> buf;
-    {
-        char tmp[1 << 16];
-        size_t n;
-        while ((n = fread(tmp, 1, sizeof(tmp), fp)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+    if (fseek(fp, 0, SEEK_END) == 0) {
+        const long size = ftell(fp);
+        rewind(fp);
+        if (size > 0) {
+            buf.resize((size_t)size);
+            buf.resize(fread(buf.data(), 1, (size_t)size, fp));
+        }
     }
     fclose(fp);
     buf.push_back(0);
@@ -245,12 +353,7 @@ int af_case_read_dem(af_case* c, const char* path) {
     c->cellsize = cs;
     c->nodata = nod;
     c->dem.assign((size_t)ncols * nrows, 0.0);
-    size_t k = 0, n = c->dem.size();
-    for (; k < n; k++) {
-        double v;
-        if (!read_dbl(&v)) break;  // fscanf fails from here on
-        c->dem[k] = v;
-    }
+    parse_values(s, buf.data() + buf.size() - 1, c->dem);
     c->have_dem = true;
     return AF_OK;
 }
@@ -336,16 +439,28 @@ int af_case_read_bdy(af_case* c, const char* path) {
         af::set_error("Cannot open BDY file %s", path);
         return AF_E_IO;
     }
+    // A section header is a line that starts with a point's name followed by
+    // a space, CR or the end of the line; the first QVAR/HVAR point (in file
+    // order) with that name takes the series (io.cpp:266-289).  Names hold
+    // no whitespace, so that is the line's text up to its first space or CR
+    // looked up among the names: one hash lookup per line instead of a scan
+    // of every point.
+    std::unordered_map<std::string, size_t> first;
+    for (size_t k = 0; k < c->points.size(); k++) {
+        const Point& pb = c->points[k];
+        if ((pb.type != AF_BC_QVAR && pb.type != AF_BC_HVAR) || pb.name.empty()) continue;
+        first.emplace(pb.name, k);
+    }
     std::string line;
     std::getline(f, line);
     double val = 0.0, time = 0.0;
     while (std::getline(f, line)) {
         if (line.empty() || line[0] == '#') continue;
-        for (Point& pb : c->points) {
-            if (pb.type != AF_BC_QVAR && pb.type != AF_BC_HVAR) continue;
-            const size_t L = pb.name.size();
-            if (line.compare(0, L, pb.name) == 0 && line.size() >= L &&
-                (line.size() == L || line[L] == ' ' || line[L] == '\n' || line[L] == '\r')) {
+        const size_t cut = line.find_first_of(" \n\r");
+        const auto hit = first.find(cut == std::string::npos ? line : line.substr(0, cut));
+        if (hit != first.end()) {
+            Point& pb = c->points[hit->second];
+            {
                 if (!std::getline(f, line)) line.clear();
                 int n = 0;
                 if (sscanf(line.c_str(), "%d", &n) != 1) n = 0;
@@ -358,7 +473,6 @@ int af_case_read_bdy(af_case* c, const char* path) {
                     pb.times.push_back(time);
                     pb.values.push_back(val);
                 }
-                break;
             }
         }
     }
@@ -625,12 +739,44 @@ int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nr
     fprintf(fp, "yllcorner     %.1f\n", yll);
     fprintf(fp, "cellsize      %.1f\n", cellsize);
     fprintf(fp, "NODATA_value  -9999\n");
-    std::vector<char> row;
-    for (int32_t j = 0; j < nrows; j++) {
-        for (int32_t i = 0; i < ncols; i++) fprintf(fp, "%.4f ", data[i + (long long)j * ncols]);
-        fprintf(fp, "\n");
+    // rows are formatted ("%.4f " per value, a newline per row) by several
+    // threads into per-block buffers and written in order
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 1;
+    if (nt > 16) nt = 16;
+    if ((long long)ncols * nrows < (1 << 20)) nt = 1;
+    const int32_t rows_per_batch = (int32_t)std::max<long long>(1, (1LL << 22) / ncols) * nt;
+    std::vector<std::string> blk(nt);
+    for (int32_t j0 = 0; j0 < nrows; j0 += rows_per_batch) {
+        const int32_t j1 = std::min(nrows, j0 + rows_per_batch);
+        auto work = [&](unsigned t) {
+            const int32_t a = j0 + (int32_t)((long long)(j1 - j0) * t / nt);
+            const int32_t b = j0 + (int32_t)((long long)(j1 - j0) * (t + 1) / nt);
+            std::string& o = blk[t];
+            o.clear();
+            char tmp[64];
+            for (int32_t j = a; j < b; j++) {
+                for (int32_t i = 0; i < ncols; i++) {
+                    const int len = fmt4(tmp, sizeof tmp, data[i + (long long)j * ncols]);
+                    o.append(tmp, (size_t)len);
+                }
+                o.push_back('\n');
+            }
+        };
+        if (nt == 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nt; t++) th.emplace_back(work, t);
+            for (auto& x : th) x.join();
+        }
+        for (unsigned t = 0; t < nt; t++) fwrite(blk[t].data(), 1, blk[t].size(), fp);
     }
-    fclose(fp);
+    const bool ok = (fclose(fp) == 0);
+    if (!ok) {
+        af::set_error("write error on %s", path);
+        return AF_E_IO;
+    }
     return AF_OK;
 }
 

@@ -154,6 +154,17 @@ def test_bdy_sections_prefix_and_first_match(tmp_path):
     assert list(pts[3].values) == [0.3]                          # CR after the name matches
 
 
+def test_bdy_header_needs_space_cr_or_end(tmp_path):
+    """A name followed by a tab or by more characters is not a section
+    header (only ' ', CR or the end of the line end the name)."""
+    c = _case_with_dem(tmp_path)
+    c.read_bci(_w(tmp_path, "q.bci", "P 1015 2075 QVAR in1\n"))
+    c.read_bdy(_w(tmp_path, "q.bdy", "header\nin1\t2\n1 0\nin1x\n1 seconds\n9 0\n"
+                                     "in1\n1 s\n4.0 30\n"))
+    pts = c.points()
+    assert list(pts[0].values) == [4.0] and list(pts[0].times) == [30.0]
+
+
 def test_rain_file(tmp_path):
     c = formats.Case()
     c.read_rain(_w(tmp_path, "r.txt", "rate time\n1e-5 0\n# c\n\n2e-5 600\nbad line\n"))
@@ -234,3 +245,67 @@ def test_missing_inputs_fail_cleanly(tmp_path):
     par = _w(tmp_path, "p.par", f"DEMfile {tmp_path}/nope.asc\n")
     with pytest.raises(AFError):
         formats.load_case(par)
+
+
+def test_large_dem_parallel_parse_and_stop_rule(tmp_path):
+    """A DEM large enough to be parsed in pieces by several threads gives the
+    values a sequential fscanf loop gives, including the stop rule: from the
+    first token that does not convert, every later cell stays 0."""
+    rng = np.random.default_rng(5)
+    ny, nx = 300, 1000
+    z = np.round(rng.normal(50.0, 20.0, (ny, nx)), 3)
+    lines = [f"ncols {nx}", f"nrows {ny}", "xllcorner 0", "yllcorner 0", "cellsize 5",
+             "NODATA_value -9999"]
+    body = [" ".join(repr(float(v)) for v in row) for row in z]
+    p = tmp_path / "big.dem.ascii"
+    p.write_text("\n".join(lines + body) + "\n")
+    assert p.stat().st_size > (1 << 20)  # above the one-thread cut-off
+    c = formats.Case()
+    c.read_dem(str(p))
+    np.testing.assert_array_equal(c.scenario().dem, z)
+    # a bad token in row 200: everything from there on reads 0
+    body[200] = body[200].replace(repr(float(z[200, 500])), "x7", 1)
+    p.write_text("\n".join(lines + body) + "\n")
+    c = formats.Case()
+    c.read_dem(str(p))
+    want = z.copy().ravel()
+    want[200 * nx + 500:] = 0.0
+    np.testing.assert_array_equal(c.scenario().dem.ravel(), want)
+
+
+def test_large_asc_write_matches_printf(tmp_path):
+    """The threaded writer's bytes equal a sequential '%.4f ' per value, one
+    line per row, under the reference header."""
+    rng = np.random.default_rng(6)
+    ny, nx = 260, 4100
+    h = rng.exponential(0.3, (ny, nx))
+    h[rng.random((ny, nx)) < 0.3] = 0.0
+    p = tmp_path / "H.asc"
+    formats.write_asc(str(p), h, xll=10.0, yll=20.0, cellsize=2.0)
+    text = p.read_text()
+    head = ("ncols         4100\nnrows         260\nxllcorner     10.0\nyllcorner     20.0\n"
+            "cellsize      2.0\nNODATA_value  -9999\n")
+    assert text.startswith(head)
+    rows = text[len(head):].split("\n")
+    assert len(rows) == ny + 1 and rows[-1] == ""
+    for j in (0, 1, 129, 259):
+        assert rows[j] == "".join("%.4f " % v for v in h[j])
+
+
+def test_asc_value_format_matches_printf_at_the_edges(tmp_path):
+    """Values around every rounding edge of '%.4f': exact binary halves
+    (ties to even), values one ulp either side of them, -0.0 and tiny
+    negatives (printed '-0.0000'), large values, inf and NaN."""
+    import math
+    vals = [0.0, -0.0, -1e-9, 1e-9, 1.03125, 2.00005, 0.00005, 123.45675, -7.77775, 9.99995,
+            99999.99995, 1e5 - 1e-11, 1e10 + 0.5, 3e11, -2.5e12, math.inf, -math.inf, math.nan]
+    for v in (1.03125, 0.15625, 2.84375, 100.00005):
+        vals += [math.nextafter(v, math.inf), math.nextafter(v, -math.inf)]
+    rng = np.random.default_rng(9)
+    vals += list(np.round(rng.uniform(-50, 50, 2000), 5))  # many near-halves
+    vals += list(rng.exponential(1.0, 2000) * rng.choice([-1, 1], 2000))
+    a = np.array(vals, dtype=np.float64)[None, :]
+    p = tmp_path / "v.asc"
+    formats.write_asc(str(p), a)
+    row = p.read_text().split("\n")[6]
+    assert row == "".join("%.4f " % v for v in a[0])
