@@ -442,6 +442,40 @@ def bench_divergence(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_shard(args):
+    """One GPU's share of the 16384^2 roofline case split into `--parts` row
+    slabs: slab 0 (16384 x 2048 at 8 parts) with its own point sources.  The
+    slab's north and south edges stand in for the halo rows a sharded run
+    would exchange; the step's work per cell is the same."""
+    from adaptflood_amd import generators
+    from adaptflood_amd.scenario import Scenario
+    full = generators.roofline_case(n=args.size)
+    ny = full.ny // args.parts
+    pts = [p for p in full.points if p.y < ny]
+    slab = Scenario(dem=full.dem[:ny].copy(), dx=full.dx, sim_time=full.sim_time,
+                    west_free=True, east_free=True, north_free=True, south_free=False,
+                    points=pts, name=f"slab {full.nx}x{ny}")
+    del full
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    cells = slab.cells
+    res = {}
+    for prec, B in (("fp32", 28.0), ("fp64", 56.0)):
+        wall, kern, _ = measure(slab, prec, args.steps, args.warmup, args.tile_rows, args.wpb)
+        res[prec] = {"value": round(cells * args.steps / wall / 1e9, 3), "unit": "GCUPS",
+                     "ms_per_step": round(wall * 1e3 / args.steps, 4),
+                     "roofline": {"bound": "hbm", "achieved": round(B * cells / kern / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(B * cells / kern / 1e9 / HBM_PEAK_GBS, 4),
+                                  "kernel_ms": round(kern * 1e3, 4)}}
+        print(f"[bench] shard {prec}: {res[prec]}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "GCUPS of one GPU's row slab", "config": {
+        "workload": f"row slab {slab.nx}x{slab.ny} (1/{args.parts} of the {args.size}^2 "
+                    f"roofline case), {len(pts)} QFIX", "cells": cells}, **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -462,8 +496,9 @@ def main():
                     help="also run a 4 GiB device-to-device copy (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
-                    choices=("roofline", "ensemble", "served", "accuracy", "divergence"),
+                    choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard"),
                     default="roofline")
+    ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
     ap.add_argument("--acc-time", type=float, default=600.0,
                     help="accuracy: simulated seconds of the oracle and roofline comparisons")
@@ -495,6 +530,8 @@ def main():
         return bench_accuracy(args)
     if args.workload == "divergence":
         return bench_divergence(args)
+    if args.workload == "shard":
+        return bench_shard(args)
     served_reqs = None
     if not args.no_served:
         # generated in worker processes before this process touches the GPU
