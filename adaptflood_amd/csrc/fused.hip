@@ -552,13 +552,21 @@ int build_graph(af_domain* D, hipStream_t st) {
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
-    if (rc != AF_OK) return rc;
-    if (e != hipSuccess) {
+    if (rc != AF_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != AF_OK) return rc;
         set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
         return AF_E_HIP;
     }
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+        return AF_E_HIP;
+    }
     D->graph = g;
-    AF_HIP(hipGraphInstantiate(&D->gexec, g, nullptr, nullptr, 0));
+    D->gexec = ge;
     (void)st;
     return AF_OK;
 }

@@ -25,6 +25,11 @@
  * Every entry point returns an int status: AF_OK (0) or a negative AF_E_*
  * code; af_last_error() gives a message for the calling thread.  The library
  * never allocates device memory: the caller owns every device buffer.
+ *
+ * Threads: the per-function entry points keep no state between calls and
+ * may be called from several threads at once (each with its own buffers and
+ * FloodplainQ workspace).  A domain, batch or pool (af_domain) is not
+ * thread-safe: calls on one af_domain must not overlap.
  */
 #ifndef ADAPTFLOOD_AMD_H
 #define ADAPTFLOOD_AMD_H
