@@ -179,6 +179,14 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
         else other.push_back(p);
     }
     auto idx_of = [&](int p) { return d.pt_x[p] + d.pt_y[p] * nx; };
+    // the per-function kernels index H with these directly (as main.cpp:
+    // 253-259 and kernels.cpp:263-272 do); a point outside the grid is an error
+    for (int p = 0; p < d.n_points; p++) {
+        const long long k = (long long)d.pt_x[p] + (long long)d.pt_y[p] * nx;
+        if (k < 0 || k >= (long long)nx * ny)
+            throw Fail{"point " + std::to_string(p) + " at grid (" + std::to_string(d.pt_x[p]) +
+                       ", " + std::to_string(d.pt_y[p]) + ") lies outside the grid"};
+    }
     auto value_at = [&](int p, double t) {
         const int len = d.pt_series_len ? d.pt_series_len[p] : 0;
         const int off = d.pt_series_off ? d.pt_series_off[p] : 0;

@@ -90,7 +90,9 @@ int af_BCs_GPU(double* Qx, double* Qy, const double* H, const double* DEM, int x
 /* Replaces ApplyPointBCs_GPU (src/kernels.hpp:24-25, src/kernels.cpp:259-277):
  * bc_type 1 (FREE) sets H = 0, 2/3 (HFIX/HVAR) set H = bc_value.  Points that
  * share a cell resolve in array order (the last one wins).  ncols, dx, dA and
- * Tstep are unused, as in the reference. */
+ * Tstep are unused, as in the reference.  Every bc_idx must lie in
+ * [0, xsz*ysz): like the reference, the call does not know the grid size.
+ * Duplicates are found by a scan of the point list (O(num_bcs^2) work). */
 int af_ApplyPointBCs_GPU(double* H, const int32_t* bc_idx, const int32_t* bc_type,
                          const double* bc_value, int num_bcs, int ncols, double dx, double dA,
                          double Tstep, af_stream_t stream);
@@ -107,7 +109,8 @@ int af_Rainfall_GPU(double* H, const double* DEM, int xsz, int ysz, double rain_
 
 /* Replaces the inline AddFlowBCs kernel (src/main.cpp:256-260):
  * H[ps_idx[s]] += ps_flow[s], clamped at 0.  Sources that share a cell are
- * applied in array order. */
+ * applied in array order.  Every ps_idx must lie in [0, xsz*ysz).  Duplicates
+ * are found by a scan of the source list (O(num_flow_bcs^2) work). */
 int af_AddFlowBCs_GPU(double* H, const int32_t* ps_idx, const double* ps_flow, int num_flow_bcs,
                       af_stream_t stream);
 
