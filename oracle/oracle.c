@@ -20,6 +20,7 @@
  *   orc_interp             src/io.cpp:299-311 and io.cpp:88-100 (interpolation)
  *   orc_run                src/main.cpp:196-304     (time loop + mass report)
  *   orc_velocity           src/kernels.cpp:293-351  (ComputeVelocity_GPU)
+ *   orc_pow_10_3           src/kernels.cpp:83, 133  (pow(hflow, 10.0/3.0))
  *
  * Layout (reference): cells idx = i + j*xsz (row 0 = north); face arrays Qx,
  * Qy, Qxold, Qyold have (xsz+1)*(ysz+1) entries with pitch xsz+1.
@@ -34,8 +35,17 @@
  * single-face flux evaluated independently in long double, per-edge BC closed
  * forms, source volume v*dx*dt, rain volume r*t*A_valid on a closed basin,
  * lake-at-rest, closed-domain mass conservation, mirror symmetry, Manning
- * normal depth on a plane).  The reference itself cannot be built in this
+ * normal depth on a plane, a hand-computed velocity field, and orc_pow_10_3
+ * against the C library's pow and against 80-digit decimal values);
+ * tests/test_golden.py holds the oracle to the committed trajectory of the
+ * 256 x 256 plane workload.  The reference itself cannot be built in this
  * environment (it needs Kokkos), so no vector from a reference run exists.
+ *
+ * pow: the reference takes pow(hflow, 10.0/3.0) from its backend's libm
+ * (kernels.cpp:83, :133).  The oracle uses orc_pow_10_3, a pow built from
+ * IEEE-exact operations that is within 1 ulp of glibc's everywhere, so that
+ * a device implementation of the same operation sequence agrees with the
+ * oracle bit for bit; see orc_pow_10_3 below.
  */
 #include <math.h>
 #include <float.h>
