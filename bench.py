@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="CPU oracle steps of the baseline sample (about 15 s on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the reference-call-sequence timing of the default workload")
     ap.add_argument("--no-served", action="store_true",
                     help="skip the served-ensemble latency run of the default workload")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -609,6 +611,20 @@ def main():
                                     "frac": round(a64 / HBM_PEAK_GBS, 4),
                                     "traffic": _traffic_from_profile("fp64"),
                                     "kernel_ms": round(kern64 * 1e3, 4)}}
+    if not args.no_plain:
+        # the reference's own call sequence (per-function FP64 kernels, dt read
+        # back every step) on the same grid, for scale
+        from adaptflood_amd import kernels
+        r = kernels.run_sequence(scn, max_steps=args.plain_steps)
+        g = cells * r["steps"] / r["loop_seconds"] / 1e9
+        out["plain_path"] = {"value": round(g, 3), "unit": "GCUPS",
+                             "ms_per_step": round(r["loop_seconds"] * 1e3 / r["steps"], 3),
+                             "sample": f"reference call sequence through the per-function C "
+                                       f"ABI, FP64, first {r['steps']} steps of the same workload"}
+        print(f"[bench] plain path fp64: {g:.2f} GCUPS, "
+              f"{r['loop_seconds'] * 1e3 / r['steps']:.2f} ms/step", file=sys.stderr, flush=True)
+        del r
+        torch.cuda.empty_cache()
     if served_reqs is not None:
         reqs, arr = served_reqs
         done, summary, budget = run_served(reqs, arr, "fp32", args.budget_gb, args.chunk)
