@@ -82,6 +82,9 @@ class Pool:
     def advance(self, steps, graph=True):
         call("af_domain_advance", self.handle, int(steps), 1 if graph else 0, self._s())
 
+    def synchronize(self):
+        self._torch.cuda.synchronize()
+
     def poll(self):
         """(finished slots, number of active slots)."""
         cap = max(1, len(self.packed))
@@ -128,11 +131,14 @@ class Request:
 def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, graph=True):
     """Runs requests (a list of Request, sorted by arrival) through the pool.
 
+    The pool needs admit(scenario) -> slot or None, release(slot),
+    advance(steps, graph), poll() -> (finished slots, n_active),
+    info(slot) -> {"steps": ...}, read(slot, "H") and synchronize().
+
     clock="wall": arrivals are seconds on the wall clock from the start of the
     call (a request is admitted once it has arrived); clock="steps": arrivals
     are pool step indices (a deterministic schedule for tests).  Returns the
     requests with admission/completion times filled in and a summary."""
-    torch = pool._torch
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
     waiting = collections.deque()
     active = {}
@@ -176,7 +182,7 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
             r.done = now()
             pool.release(slot)
             done.append(r)
-    torch.cuda.synchronize()
+    pool.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([r.done - r.arrival for r in done]) if done else np.zeros(1)
     summary = {

@@ -1,0 +1,81 @@
+"""The continuous-batching request loop (adaptflood_amd.serve.serve) on a
+stand-in pool that simulates admission capacity and step counts on the CPU:
+admission order, waiting while full, release and re-admission, completion
+times and the latency summary."""
+from adaptflood_amd.serve import Request, serve
+
+
+class Scn:
+    def __init__(self, cells, steps):
+        self.cells, self.steps = cells, steps
+
+
+class FakePool:
+    """Holds up to `slots` scenarios; each finishes after its own number of
+    steps; every call is logged."""
+
+    def __init__(self, slots):
+        self.slots, self.active, self.log, self.step = slots, {}, [], 0
+
+    def admit(self, scn):
+        if len(self.active) >= self.slots:
+            return None
+        slot = min(set(range(self.slots)) - set(self.active))
+        self.active[slot] = [scn, 0]
+        self.log.append(("admit", self.step, slot))
+        return slot
+
+    def release(self, slot):
+        self.log.append(("release", self.step, slot))
+        del self.active[slot]
+
+    def advance(self, steps, graph=True):
+        self.step += steps
+        for rec in self.active.values():
+            rec[1] = min(rec[0].steps, rec[1] + steps)
+
+    def poll(self):
+        done = [s for s, (scn, k) in sorted(self.active.items()) if k >= scn.steps]
+        return done, len(self.active)
+
+    def info(self, slot):
+        return {"steps": self.active[slot][1]}
+
+    def read(self, slot, what="H"):
+        return ("result", self.active[slot][0].steps)
+
+    def synchronize(self):
+        pass
+
+
+def test_admission_waits_for_free_slots_and_keeps_arrival_order():
+    # five requests, two slots; arrivals on the step clock
+    steps = [10, 40, 20, 5, 30]
+    reqs = [Request(k, Scn(100 * (k + 1), steps[k]), arrival=a)
+            for k, a in enumerate([0, 0, 3, 4, 50])]
+    pool = FakePool(slots=2)
+    done, summary = serve(reqs, pool, chunk_steps=5, clock="steps", keep_results=True)
+    assert summary["requests"] == 5
+    admits = [e for e in pool.log if e[0] == "admit"]
+    # requests 0 and 1 at step 0; 2 once request 0 is released; 3 after that
+    assert [e[1] for e in admits][:2] == [0, 0]
+    by_id = {r.rid: r for r in done}
+    assert by_id[2].admitted >= by_id[0].done
+    assert by_id[3].admitted >= by_id[2].admitted
+    # every request ran exactly its own number of steps, and its result came back
+    for r in done:
+        assert r.steps == steps[r.rid]
+        assert r.result == ("result", steps[r.rid])
+        assert r.done >= r.arrival and r.admitted >= r.arrival
+    assert summary["cell_updates"] == sum(100 * (k + 1) * steps[k] for k in range(5))
+    lat = sorted(r.done - r.arrival for r in done)
+    assert summary["p50_latency"] == lat[2]
+
+
+def test_idle_pool_jumps_to_the_next_arrival():
+    reqs = [Request(0, Scn(10, 5), arrival=0), Request(1, Scn(10, 5), arrival=1000)]
+    pool = FakePool(slots=4)
+    done, summary = serve(reqs, pool, chunk_steps=5, clock="steps")
+    assert [r.rid for r in done] == [0, 1]
+    assert done[1].admitted == 1000
+    assert pool.step < 1100   # no stepping through the idle gap
