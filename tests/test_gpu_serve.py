@@ -50,3 +50,22 @@ def test_pool_capacity_and_release(gpu):
     pool.advance(12, graph=False)
     fin, act = pool.poll()
     assert act == 2
+
+
+def test_served_fp64_equals_oracle_default_tiling(gpu):
+    """Served in a mix with the default, per-scenario tile heights (widths a
+    whole number of strips and not), every FP64 result is bitwise the
+    oracle's."""
+    from oracle import oracle
+    sizes = [(256, 40), (100, 64), (512, 24), (33, 130)]
+    scns = [generators.small_case(nx, ny, seed=90 + k, sim_time=40.0 + 20 * k)
+            for k, (nx, ny) in enumerate(sizes)]
+    reqs = [Request(k, s, arrival=5 * k) for k, s in enumerate(scns)]
+    pool = Pool(budget_bytes=4 << 20, precision="fp64", max_scenarios=2, max_tiles=4096)
+    done, summary = serve(reqs, pool, chunk_steps=6, clock="steps", keep_results=True)
+    assert summary["requests"] == len(scns)
+    for r in done:
+        ref = oracle.run(r.scenario)
+        print(f"request {r.rid}: {r.scenario.nx}x{r.scenario.ny} steps={r.steps}", flush=True)
+        assert r.steps == ref["steps"], r.rid
+        np.testing.assert_array_equal(r.result, ref["H"])
