@@ -116,6 +116,15 @@ class Batch:
         call("af_batch_read", self.handle, i, which, _lib.dptr(out), self._s())
         return out
 
+    def read_cells(self, idx, i=0):
+        """H at flat cell indices j*nx + i of scenario i (the stage gauges of
+        main.cpp:306-322), as float64; NaN for an index outside the grid."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64).ravel()
+        out = np.empty(max(idx.size, 1), dtype=np.float64)
+        call("af_batch_read_cells", self.handle, i,
+             idx.ctypes.data_as(C.POINTER(C.c_int64)), int(idx.size), _lib.dptr(out), self._s())
+        return out[: idx.size]
+
     def dt_log(self, i=0):
         n = C.c_int64()
         call("af_batch_read_dt_log", self.handle, i, None, 0, C.byref(n), self._s())

@@ -210,3 +210,22 @@ def test_autotune_picks_a_candidate_and_caches(gpu, tmp_path):
     again, table2 = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), cache_path=cache)
     assert (again.tile_rows, again.variant) == (tun.tile_rows, tun.variant)
     assert len(table2) == 4
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_read_cells_equals_full_read(gpu, prec):
+    """The gauge read (read_cells) returns the same values as a full read of H
+    at those cells, for both state parities, and NaN outside the grid."""
+    scn = _small(61, 300, 37, sim_time=90.0)
+    d = Domain(scn, precision=prec, tuning=make_tuning(tile_rows=8))
+    n = scn.nx * scn.ny
+    idx = np.array([0, 1, scn.nx - 1, scn.nx, n // 2 + 7, n - 1, -1, n, 5 * scn.nx + 3],
+                   dtype=np.int64)
+    for steps in (3, 4):   # odd and even step counts end on different buffers
+        d.advance(steps, graph=False)
+        got = d.read_cells(idx)
+        H = d.read("H").ravel()
+        inside = (idx >= 0) & (idx < n)
+        assert np.array_equal(got[inside], H[idx[inside]])
+        assert np.all(np.isnan(got[~inside]))
+    assert d.read_cells(np.empty(0, dtype=np.int64)).size == 0

@@ -69,3 +69,28 @@ def test_served_fp64_equals_oracle_default_tiling(gpu):
         print(f"request {r.rid}: {r.scenario.nx}x{r.scenario.ny} steps={r.steps}", flush=True)
         assert r.steps == ref["steps"], r.rid
         np.testing.assert_array_equal(r.result, ref["H"])
+
+
+def test_served_fp32_strip_widths_equals_alone(gpu):
+    """FP32 served with the default tile heights: widths a whole number of
+    256-column strips, one column more, and narrow ones, admitted while others
+    are mid-run (two slots); each result and step count is bitwise the
+    scenario's alone, and within the FP32 bar of the oracle."""
+    from oracle import oracle
+    sizes = [(256, 40), (512, 33), (257, 20), (100, 64), (768, 9)]
+    scns = [generators.small_case(nx, ny, seed=110 + k, sim_time=50.0 + 15 * k)
+            for k, (nx, ny) in enumerate(sizes)]
+    reqs = [Request(k, s, arrival=4 * k) for k, s in enumerate(scns)]
+    pool = Pool(budget_bytes=4 << 20, precision="fp32", max_scenarios=2, max_tiles=4096)
+    done, summary = serve(reqs, pool, chunk_steps=5, clock="steps", keep_results=True)
+    assert summary["requests"] == len(scns)
+    for r in done:
+        d = Domain(r.scenario, precision="fp32")
+        d.run(check_every=20, graph=True)
+        ref = oracle.run(r.scenario)
+        err = float(np.max(np.abs(r.result - ref["H"])))
+        print(f"request {r.rid}: {r.scenario.nx}x{r.scenario.ny} steps={r.steps} "
+              f"max|dH| vs oracle={err:.2e}", flush=True)
+        assert r.steps == d.info()["steps"], r.rid
+        assert np.array_equal(r.result, d.read("H")), r.rid
+        assert err <= 1e-3
