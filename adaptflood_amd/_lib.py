@@ -106,6 +106,7 @@ SIGNATURES = [
     ("af_batch_set_sim_time", _I, [_V, _I, C.c_double, _V]),
     ("af_batch_read_cells", _I, [_V, _I, C.POINTER(_I64), _I, _dp, _V]),
     ("af_batch_read", _I, [_V, _I, _I, _dp, _V]),
+    ("af_batch_snapshot", _I, [_V, _I, _V, _V, _V, _V]),
     ("af_batch_read_dt_log", _I, [_V, _I, _dp, _I64, C.POINTER(_I64), _V]),
     ("af_batch_read_reports", _I, [_V, _I, _dp, _I32, C.POINTER(_I32), _V]),
     ("af_batch_volume", _I, [_V, _I, _dp, _dp, _V]),
@@ -138,6 +139,7 @@ SIGNATURES = [
     ("af_case_stage", _I, [_V, _I32, _ip, _ip, C.c_char_p, _I32, _ip]),
     ("af_interp_series", _D, [_dp, _dp, _I32, _D, _D]),
     ("af_write_asc", _I, [C.c_char_p, _dp, _I32, _I32, _D, _D, _D]),
+    ("af_write_asc_async", _I, [C.c_char_p, _dp, _I32, _I32, _D, _D, _D, _D]),
 ]
 
 _lib = None

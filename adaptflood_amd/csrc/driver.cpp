@@ -15,15 +15,28 @@
 // at t = 0 and at every save, H_final.asc, Vx_final.asc / Vy_final.asc when
 // the .par file asks for velocity output (main.cpp:380-392), and stage.csv
 // when it names a stage file (main.cpp:95-113, 306-322).
+//
+// With async_output on (the default, types.hpp:37) the saves go through a
+// writer thread as in AsyncOutputManager (async_output.cpp): "Queued:" lines,
+// files in that writer's header format, Vx_%04d/Vy_%04d.asc beside H_%04d.asc
+// when velocity output is on, and no ASC saves for output_format 1 (NetCDF,
+// which this build does not write).  Each save is a device snapshot into one
+// of two slots, copied to pinned host memory on a side stream while the
+// steps go on, and written by the thread.
 #include <hip/hip_runtime_api.h>
 #include <sys/stat.h>
 
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "adaptflood.h"
@@ -109,6 +122,166 @@ struct Output {
     }
 };
 
+// The asynchronous writer.  A save takes a free slot (waiting for the
+// thread when both are busy), fills its device buffers on the compute stream
+// (a snapshot), and submits it: the side stream waits for the snapshot,
+// copies the buffers to the slot's pinned host memory, and the thread waits
+// for that copy, writes the files and frees the slot.
+class AsyncWriter {
+  public:
+    struct Slot {
+        double* dev[3] = {nullptr, nullptr, nullptr};   // H, Vx, Vy (device)
+        double* host[3] = {nullptr, nullptr, nullptr};  // pinned copies
+        hipEvent_t snap = nullptr, copied = nullptr;
+        bool busy = false;
+    };
+
+    AsyncWriter(const Output& out, bool velocity, bool asc)
+        : out_(out), velocity_(velocity), asc_(asc), n_((size_t)out.g.nx * out.g.ny) {
+        hip_ok(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
+        const int fields = velocity_ ? 3 : 1;
+        for (Slot& s : slots_) {
+            for (int f = 0; f < fields; f++) {
+                hip_ok(hipMalloc(&s.dev[f], n_ * 8), "hipMalloc snapshot");
+                hip_ok(hipHostMalloc(&s.host[f], n_ * 8, hipHostMallocDefault),
+                       "hipHostMalloc snapshot");
+            }
+            hip_ok(hipEventCreateWithFlags(&s.snap, hipEventDisableTiming), "hipEventCreate");
+            hip_ok(hipEventCreateWithFlags(&s.copied, hipEventDisableTiming), "hipEventCreate");
+        }
+        thread_ = std::thread(&AsyncWriter::loop, this);
+    }
+
+    ~AsyncWriter() {
+        stop();
+        for (Slot& s : slots_) {
+            for (int f = 0; f < 3; f++) {
+                if (s.dev[f]) (void)hipFree(s.dev[f]);
+                if (s.host[f]) (void)hipHostFree(s.host[f]);
+            }
+            if (s.snap) (void)hipEventDestroy(s.snap);
+            if (s.copied) (void)hipEventDestroy(s.copied);
+        }
+        if (copy_) (void)hipStreamDestroy(copy_);
+    }
+
+    bool velocity() const { return velocity_; }
+
+    // A free slot; waits while the thread still holds both.
+    Slot& acquire() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            for (Slot& s : slots_)
+                if (!s.busy) {
+                    s.busy = true;
+                    return s;
+                }
+            cv_.wait(lk);
+        }
+    }
+
+    // The slot's device buffers are (being) filled on `compute`.
+    void submit(Slot& s, int save_num, hipStream_t compute) {
+        hip_ok(hipEventRecord(s.snap, compute), "hipEventRecord");
+        hip_ok(hipStreamWaitEvent(copy_, s.snap, 0), "hipStreamWaitEvent");
+        const int fields = velocity_ ? 3 : 1;
+        for (int f = 0; f < fields; f++)
+            hip_ok(hipMemcpyAsync(s.host[f], s.dev[f], n_ * 8, hipMemcpyDeviceToHost, copy_),
+                   "snapshot copy");
+        hip_ok(hipEventRecord(s.copied, copy_), "hipEventRecord");
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back({&s, save_num});
+        }
+        cv_.notify_all();
+    }
+
+    // Waits until every submitted save is written (AsyncOutputManager::finish);
+    // fails if a write failed.
+    void finish() {
+        stop();
+        if (!error_.empty()) throw Fail{error_};
+    }
+
+  private:
+    struct Job {
+        Slot* slot;
+        int save_num;
+    };
+
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (stopping_) return;
+            stopping_ = true;
+        }
+        cv_.notify_all();
+        if (thread_.joinable()) thread_.join();
+    }
+
+    void loop() {
+        for (;;) {
+            Job job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !jobs_.empty() || stopping_; });
+                if (jobs_.empty()) return;
+                job = jobs_.front();
+                jobs_.pop_front();
+            }
+            write(job);
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                job.slot->busy = false;
+            }
+            cv_.notify_all();
+        }
+    }
+
+    void write(const Job& job) {
+        const hipError_t e = hipEventSynchronize(job.slot->copied);
+        if (e != hipSuccess) {
+            note(std::string("snapshot copy: ") + hipGetErrorString(e));
+            return;
+        }
+        if (!asc_) return;  // output_format 1: NetCDF only (async_output.cpp:144)
+        static const char* names[3] = {"H", "Vx", "Vy"};
+        const int fields = velocity_ ? 3 : 1;
+        for (int f = 0; f < fields; f++) {
+            char name[64];
+            snprintf(name, sizeof name, "%s_%04d.asc", names[f], job.save_num);
+            const std::string path = out_.dir + "/" + name;
+            if (af_write_asc_async(path.c_str(), job.slot->host[f], out_.g.nx, out_.g.ny,
+                                   out_.g.xll, out_.g.yll, out_.g.cell, out_.g.nodata) != AF_OK)
+                note(std::string("write ") + path + ": " + af_last_error());
+        }
+    }
+
+    void note(const std::string& msg) {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (error_.empty()) error_ = msg;
+    }
+
+    const Output& out_;
+    const bool velocity_, asc_;
+    const size_t n_;
+    hipStream_t copy_ = nullptr;
+    Slot slots_[2];
+    std::deque<Job> jobs_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stopping_ = false;
+    std::string error_;
+    std::thread thread_;
+};
+
+void queued_line(const Output& out, double t, int save_num) {
+    if (out.quiet) return;
+    if (save_num == 0) printf("Queued: t=0s (initial state)\n");
+    else printf("Queued: t=%.0fs (save_num=%d)\n", t, save_num);
+    fflush(stdout);
+}
+
 void report_line(double t, double dt, double vol, double area) {
     printf("t=%.0fs: Tstep=%.2f, Vol=%.0fm3, Area=%.0fm2\n", t, dt, vol, area);
     fflush(stdout);
@@ -156,7 +329,8 @@ struct Result {
 };
 
 // The reference's time loop through the drop-in wrappers (main.cpp:196-324).
-Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& out) {
+Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& out,
+                  AsyncWriter* aw) {
     const int nx = d.nx, ny = d.ny;
     const size_t n = (size_t)nx * ny, nq = (size_t)(nx + 1) * (ny + 1);
     const double dx = d.dx, dA = dx * dx;
@@ -216,12 +390,28 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
     std::vector<double> Hh(n);
     auto pull_H = [&]() { hip_ok(hipMemcpy(Hh.data(), H, n * 8, hipMemcpyDeviceToHost), "H"); };
     int save_num = 0;
-    char name[64];
-    pull_H();
-    snprintf(name, sizeof name, "H_%04d.asc", save_num++);
-    out.asc(name, Hh, " (initial state)");
+    double t = 0.0;
+    // main.cpp:206-225 (t = 0) and 326-363
+    auto save = [&]() {
+        if (aw) {
+            AsyncWriter::Slot& sl = aw->acquire();
+            hip_ok(hipMemcpyAsync(sl.dev[0], H, n * 8, hipMemcpyDeviceToDevice, nullptr),
+                   "snapshot H");
+            if (aw->velocity())
+                ComputeVelocity_GPU(sl.dev[1], sl.dev[2], H, DEM, Qx, Qy, nx, ny, dx, d.nodata);
+            aw->submit(sl, save_num, nullptr);
+            queued_line(out, t, save_num);
+        } else {
+            char name[64];
+            pull_H();
+            snprintf(name, sizeof name, "H_%04d.asc", save_num);
+            out.asc(name, Hh, save_num == 0 ? " (initial state)" : "");
+        }
+        save_num++;
+    };
+    save();
 
-    double t = 0.0, next_save = par.save_int, next_mass = par.mass_int;
+    double next_save = par.save_int, next_mass = par.mass_int;
     double next_stage = par.stage_int;
     long long step = 0;
     while (t < d.sim_time) {
@@ -275,9 +465,7 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
             next_stage += out.stages->interval;
         }
         if (t >= next_save) {
-            pull_H();
-            snprintf(name, sizeof name, "H_%04d.asc", save_num++);
-            out.asc(name, Hh);
+            save();
             next_save += par.save_int;
         }
     }
@@ -298,7 +486,8 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
 
 // The fused domain: one launch per step, graph-replayed, stopping at each
 // save time.  Mass reports are recorded on the device at the exact steps.
-Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out, int precision) {
+Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out, int precision,
+                 AsyncWriter* aw) {
     const size_t n = (size_t)d.nx * d.ny, nq = (size_t)(d.nx + 1) * (d.ny + 1);
     af_tuning tun{};
     const double nrep = d.mass_int > 0 ? std::ceil(d.sim_time / d.mass_int) + 4 : 4;
@@ -312,10 +501,27 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
 
     std::vector<double> Hh(n), rep;
     int save_num = 0, shown = 0;
-    char name[64];
-    af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
-    snprintf(name, sizeof name, "H_%04d.asc", save_num++);
-    out.asc(name, Hh, " (initial state)");
+    af_run_info info;
+    af_ok(af_domain_info(D, &info, nullptr), "info");
+    // main.cpp:206-225 (t = 0) and 326-363
+    auto save = [&]() {
+        if (aw) {
+            AsyncWriter::Slot& sl = aw->acquire();
+            const bool v = aw->velocity();
+            af_ok(af_batch_snapshot(D, 0, sl.dev[0], v ? sl.dev[1] : nullptr,
+                                    v ? sl.dev[2] : nullptr, nullptr),
+                  "snapshot");
+            aw->submit(sl, save_num, nullptr);
+            queued_line(out, info.t, save_num);
+        } else {
+            char name[64];
+            af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
+            snprintf(name, sizeof name, "H_%04d.asc", save_num);
+            out.asc(name, Hh, save_num == 0 ? " (initial state)" : "");
+        }
+        save_num++;
+    };
+    save();
 
     auto show_reports = [&]() {
         af_run_info info;
@@ -329,8 +535,6 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
         shown = got;
     };
 
-    af_run_info info;
-    af_ok(af_domain_info(D, &info, nullptr), "info");
     double next_save = par.save_int;
     const bool gauges = out.stages && out.stages->fp;
     double next_stage = par.stage_int;
@@ -358,9 +562,7 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
             next_stage += out.stages->interval;
         }
         if (info.t >= next_save) {
-            af_ok(af_domain_read(D, 0, Hh.data(), nullptr), "read H");
-            snprintf(name, sizeof name, "H_%04d.asc", save_num++);
-            out.asc(name, Hh);
+            save();
             next_save += par.save_int;
         }
     }
@@ -417,11 +619,25 @@ int main(int argc, char** argv) {
         Stages stages;
         stages.open(c, par, d.nx);
         out.stages = &stages;
+        if (par.output_format >= 1)
+            printf("Note: output_format %d asks for NetCDF, which this build does not write%s\n",
+                   par.output_format,
+                   par.async_output && par.output_format == 1 ? " (no periodic saves)" : "");
         const auto w0 = std::chrono::steady_clock::now();
-        Result r = path == "compat" ? run_compat(d, par, out)
-                                    : run_fused(d, par, out, path == "fp64" ? AF_FP64 : AF_FP32);
+        std::unique_ptr<AsyncWriter> aw;
+        if (par.async_output)
+            aw.reset(new AsyncWriter(out, par.voutput != 0, par.output_format != 1));
+        Result r = path == "compat"
+                       ? run_compat(d, par, out, aw.get())
+                       : run_fused(d, par, out, path == "fp64" ? AF_FP64 : AF_FP32, aw.get());
         const double wall =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+        if (aw) {
+            printf("\nWaiting for async output to complete...\n");
+            fflush(stdout);
+            aw->finish();
+            printf("Async output finished.\n");
+        }
         out.asc("H_final.asc", r.H);
         if (par.voutput) {
             std::vector<double> dem(d.dem, d.dem + (size_t)d.nx * d.ny);

@@ -1074,6 +1074,77 @@ int af_batch_read_cells(af_domain* D, int s, const int64_t* idx, int n, double* 
     return AF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Output snapshot of one scenario (af_batch_snapshot): H as double and, when
+// Vx is set, the velocities of ComputeVelocity (kernels.cpp:293-351), for the
+// state entering launch k, which lives in buffer ctl->par[k & 1].  The
+// velocity is evaluated in double on the state's values, in the operation
+// order of the reference (the compat velocity kernel's order).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_snapshot(const ScnDev* __restrict__ scn, int kpar, double* __restrict__ Ho,
+           double* __restrict__ Vx, double* __restrict__ Vy) {
+    const ScnDev& S = *scn;
+    const int b = S.ctl->par[kpar];
+    const T* __restrict__ H = static_cast<const T*>(S.H[b]);
+    const T* __restrict__ z = static_cast<const T*>(S.z);
+    const T* __restrict__ Qx = static_cast<const T*>(S.Qx[b]);
+    const T* __restrict__ Qy = static_cast<const T*>(S.Qy[b]);
+    const T nod = (T)S.nodata_t;
+    const double dx = S.dx;
+    const long long n = (long long)S.nx * S.ny;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const long long j = c / S.nx, i = c - j * S.nx;
+        const long long o = j * S.P + i;
+        const double h = (double)H[o];
+        Ho[c] = h;
+        if (!Vx) continue;
+        if (z[o] == nod || h < 0.001) {
+            Vx[c] = 0.0;
+            Vy[c] = 0.0;
+            continue;
+        }
+        double hflow = h;
+        if (hflow < 0.001) hflow = 0.001;
+        const double qa = (double)Qx[o], qb = (double)Qx[o + 1];
+        const double va = (fabs(qa) > 1e-10) ? qa / dx / hflow : 0.0;
+        const double vb = (fabs(qb) > 1e-10) ? qb / dx / hflow : 0.0;
+        Vx[c] = (fabs(va) > fabs(vb)) ? va : vb;
+        const double qc = (double)Qy[o], qd = (double)Qy[o + S.P];
+        const double vc = (fabs(qc) > 1e-10) ? qc / dx / hflow : 0.0;
+        const double vd = (fabs(qd) > 1e-10) ? qd / dx / hflow : 0.0;
+        Vy[c] = (fabs(vc) > fabs(vd)) ? vc : vd;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int af_batch_snapshot(af_domain* D, int s, double* H_out, double* Vx_out, double* Vy_out,
+                      af_stream_t stream) {
+    AF_REQUIRE(D && H_out && s >= 0 && s < D->max_scn && D->scn[s].active &&
+                   ((Vx_out == nullptr) == (Vy_out == nullptr)),
+               AF_E_INVALID, "snapshot: bad arguments");
+    hipStream_t st = to_stream(stream);
+    const HostScn& h = D->scn[s];
+    const long long n = (long long)h.nx * h.ny;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 16384);
+    const int kpar = (int)(D->k & 1);
+    if (D->precision == AF_FP64)
+        hipLaunchKernelGGL(k_snapshot<double>, dim3(blocks), dim3(256), 0, st, D->d_scns + s,
+                           kpar, H_out, Vx_out, Vy_out);
+    else
+        hipLaunchKernelGGL(k_snapshot<float>, dim3(blocks), dim3(256), 0, st, D->d_scns + s, kpar,
+                           H_out, Vx_out, Vy_out);
+    AF_HIP(hipGetLastError());
+    return AF_OK;
+}
+
 int af_domain_read(af_domain* D, int which, double* out, af_stream_t stream) {
     return af_batch_read(D, 0, which, out, stream);
 }

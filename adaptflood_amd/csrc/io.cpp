@@ -724,21 +724,11 @@ double af_interp_series(const double* times, const double* values, int32_t n, do
     return values[n - 1];
 }
 
-// write_asc: fixed header layout, NODATA_value always -9999, values "%.4f ".
-int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
-                 double yll, double cellsize) {
-    if (!path || !data || ncols <= 0 || nrows <= 0) return AF_E_INVALID;
-    FILE* fp = fopen(path, "w");
-    if (!fp) {
-        af::set_error("Cannot open %s for writing", path);
-        return AF_E_IO;
-    }
-    fprintf(fp, "ncols         %d\n", ncols);
-    fprintf(fp, "nrows         %d\n", nrows);
-    fprintf(fp, "xllcorner     %.1f\n", xll);
-    fprintf(fp, "yllcorner     %.1f\n", yll);
-    fprintf(fp, "cellsize      %.1f\n", cellsize);
-    fprintf(fp, "NODATA_value  -9999\n");
+namespace {
+
+// The grid body shared by both ESRI-ASCII writers: "%.4f " per value and a
+// newline per row, north row first.
+int write_asc_body(FILE* fp, const char* path, const double* data, int32_t ncols, int32_t nrows) {
     // rows are formatted ("%.4f " per value, a newline per row) by several
     // threads into per-block buffers and written in order
     unsigned nt = std::thread::hardware_concurrency();
@@ -778,6 +768,45 @@ int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nr
         return AF_E_IO;
     }
     return AF_OK;
+}
+
+FILE* open_asc(const char* path) {
+    FILE* fp = fopen(path, "w");
+    if (!fp) af::set_error("Cannot open %s for writing", path);
+    return fp;
+}
+
+}  // namespace
+
+// write_asc (io.cpp:314-334): corners "%.1f", NODATA_value always -9999.
+int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
+                 double yll, double cellsize) {
+    if (!path || !data || ncols <= 0 || nrows <= 0) return AF_E_INVALID;
+    FILE* fp = open_asc(path);
+    if (!fp) return AF_E_IO;
+    fprintf(fp, "ncols         %d\n", ncols);
+    fprintf(fp, "nrows         %d\n", nrows);
+    fprintf(fp, "xllcorner     %.1f\n", xll);
+    fprintf(fp, "yllcorner     %.1f\n", yll);
+    fprintf(fp, "cellsize      %.1f\n", cellsize);
+    fprintf(fp, "NODATA_value  -9999\n");
+    return write_asc_body(fp, path, data, ncols, nrows);
+}
+
+// AsyncOutputManager::write_asc_file (async_output.cpp:158-176): corners
+// "%.6f", NODATA_value the grid's own value as "%.0f".
+int af_write_asc_async(const char* path, const double* data, int32_t ncols, int32_t nrows,
+                       double xll, double yll, double cellsize, double nodata) {
+    if (!path || !data || ncols <= 0 || nrows <= 0) return AF_E_INVALID;
+    FILE* fp = open_asc(path);
+    if (!fp) return AF_E_IO;
+    fprintf(fp, "ncols         %d\n", ncols);
+    fprintf(fp, "nrows         %d\n", nrows);
+    fprintf(fp, "xllcorner     %.6f\n", xll);
+    fprintf(fp, "yllcorner     %.6f\n", yll);
+    fprintf(fp, "cellsize      %.1f\n", cellsize);
+    fprintf(fp, "NODATA_value  %.0f\n", nodata);
+    return write_asc_body(fp, path, data, ncols, nrows);
 }
 
 }  // extern "C"

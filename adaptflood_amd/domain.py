@@ -125,6 +125,19 @@ class Batch:
              idx.ctypes.data_as(C.POINTER(C.c_int64)), int(idx.size), _lib.dptr(out), self._s())
         return out[: idx.size]
 
+    def snapshot(self, i=0, velocity=True):
+        """Device output snapshot of scenario i: H (ny, nx) and, with
+        velocity, the ComputeVelocity fields Vx, Vy, as float64 numpy arrays.
+        The snapshot is taken on the device behind the enqueued steps."""
+        torch = _torch()
+        s = self.scenarios[i]
+        bufs = [torch.empty((s.ny, s.nx), dtype=torch.float64, device=self.arena.device)
+                for _ in range(3 if velocity else 1)]
+        ptrs = [b.data_ptr() for b in bufs] + [None] * (3 - len(bufs))
+        call("af_batch_snapshot", self.handle, i, *ptrs, self._s())
+        (self._stream or torch.cuda.current_stream()).synchronize()
+        return [b.cpu().numpy() for b in bufs]
+
     def dt_log(self, i=0):
         n = C.c_int64()
         call("af_batch_read_dt_log", self.handle, i, None, 0, C.byref(n), self._s())

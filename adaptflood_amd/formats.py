@@ -161,6 +161,15 @@ def write_asc(path, data, xll=0.0, yll=0.0, cellsize=1.0):
     call("af_write_asc", _b(path), _lib.dptr(a), nx, ny, float(xll), float(yll), float(cellsize))
 
 
+def write_asc_async(path, data, xll=0.0, yll=0.0, cellsize=1.0, nodata=-9999.0):
+    """src/async_output.cpp:158-176: corners %.6f, the grid's NODATA value as
+    %.0f, values %.4f."""
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    ny, nx = a.shape
+    call("af_write_asc_async", _b(path), _lib.dptr(a), nx, ny, float(xll), float(yll),
+         float(cellsize), float(nodata))
+
+
 def write_case(dirpath, scn, name="case", xll=0.0, yll=0.0, rain_header="rain", extra_par=""):
     """Writes a Scenario in the reference's input formats (the inverse of
     load_case): .par, .dem.ascii, .bci, .bdy and an optional rain file.

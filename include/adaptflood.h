@@ -245,6 +245,16 @@ int af_batch_set_sim_time(af_domain* d, int scenario, double sim_time, af_stream
 int af_batch_read_cells(af_domain* d, int scenario, const int64_t* idx, int n, double* out,
                         af_stream_t stream);
 int af_batch_read(af_domain* d, int scenario, int which, double* out, af_stream_t stream);
+/* Output snapshot of `scenario` on the device, replacing the host copy of H
+ * and the ComputeVelocity_GPU call at a save (src/main.cpp:326-350; velocity
+ * of src/kernels.cpp:293-351): H as double in the reference layout (nx*ny,
+ * i + j*nx, north row first) into the device buffer H_out, and, when Vx_out
+ * and Vy_out are both non-NULL, the velocities likewise.  It is enqueued on
+ * `stream` behind the steps already enqueued there and reads the state those
+ * steps leave; it does not wait for the host, so further steps may be
+ * enqueued at once and the buffers copied out after an event on `stream`. */
+int af_batch_snapshot(af_domain* d, int scenario, double* H_out, double* Vx_out, double* Vy_out,
+                      af_stream_t stream);
 int af_batch_read_dt_log(af_domain* d, int scenario, double* out, int64_t cap, int64_t* n,
                          af_stream_t stream);
 int af_batch_read_reports(af_domain* d, int scenario, double* out, int32_t cap, int32_t* n,
@@ -333,6 +343,11 @@ double af_interp_series(const double* times, const double* values, int32_t n, do
 /* ESRI-ASCII writer in the format of write_asc (src/io.cpp:314-334). */
 int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
                  double yll, double cellsize);
+/* ESRI-ASCII writer in the format of the asynchronous writer
+ * (AsyncOutputManager::write_asc_file, src/async_output.cpp:158-176):
+ * corners "%.6f", NODATA_value the grid's own value as "%.0f". */
+int af_write_asc_async(const char* path, const double* data, int32_t ncols, int32_t nrows,
+                       double xll, double yll, double cellsize, double nodata);
 
 #ifdef __cplusplus
 }

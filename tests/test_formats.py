@@ -309,3 +309,18 @@ def test_asc_value_format_matches_printf_at_the_edges(tmp_path):
     formats.write_asc(str(p), a)
     row = p.read_text().split("\n")[6]
     assert row == "".join("%.4f " % v for v in a[0])
+
+
+def test_write_asc_async_format(tmp_path):
+    """AsyncOutputManager::write_asc_file (async_output.cpp:158-176): corners
+    %.6f, cellsize %.1f, the grid's NODATA value %.0f; the grid body is
+    write_asc's ("%.4f " per value, a newline per row)."""
+    a = np.array([[0.0, 1.23456, -0.0], [-9999.0, 2.5, 1e-5]])
+    pa, ps = str(tmp_path / "a.asc"), str(tmp_path / "s.asc")
+    formats.write_asc_async(pa, a, xll=1234.5, yll=-7.125, cellsize=2.0, nodata=-32768.4)
+    formats.write_asc(ps, a, xll=1234.5, yll=-7.125, cellsize=2.0)
+    la, ls = open(pa).read().splitlines(True), open(ps).read().splitlines(True)
+    assert la[:6] == ["ncols         3\n", "nrows         2\n", "xllcorner     1234.500000\n",
+                      "yllcorner     -7.125000\n", "cellsize      2.0\n", "NODATA_value  -32768\n"]
+    assert ls[2:4] == ["xllcorner     1234.5\n", "yllcorner     -7.1\n"]
+    assert la[6:] == ls[6:] == ["0.0000 1.2346 -0.0000 \n", "-9999.0000 2.5000 0.0000 \n"]

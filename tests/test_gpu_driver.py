@@ -102,3 +102,61 @@ def test_driver_paths(gpu, tmp_path, path, tol):
     for got, want in zip(reps, ref["reports"]):
         assert got[0] == f"{want[0]:.0f}" and got[1] == f"{want[1]:.2f}"
         assert abs(int(got[2]) - want[2]) <= 1 and got[3] == f"{want[3]:.0f}"
+
+
+def _write_small_case(tmp_path, extra):
+    scn = generators.small_case(40, 30, seed=23, sim_time=240.0)
+    scn.mass_int = 60.0
+    case = tmp_path / "case"
+    case.mkdir(exist_ok=True)
+    return formats.write_case(str(case), scn, xll=1234.5, yll=987.25, extra_par=extra)
+
+
+def _saves(out, prefix):
+    return sorted(f for f in os.listdir(out) if re.match(prefix + r"_\d{4}\.asc$", f))
+
+
+@pytest.mark.parametrize("path", ["compat", "fp32"])
+def test_async_and_sync_saves(gpu, tmp_path, path):
+    """async_output 1 (the reference default) writes the saves through the
+    writer thread in AsyncOutputManager's format (corners %.6f, the grid's
+    NODATA value) with Vx/Vy beside each H; async_output 0 writes write_asc's
+    format and H only.  The grids of every save agree to the last digit;
+    output_format 1 (NetCDF) in async mode writes no ASC saves."""
+    runs = {}
+    for mode in (1, 0):
+        sub = tmp_path / f"m{mode}"
+        sub.mkdir()
+        par = _write_small_case(sub, f"saveint 60\nvoutput\nasync_output {mode}\n")
+        r = subprocess.run([EXE, "--path", path, par], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs[mode] = (str(sub / "case" / "results"), r.stdout)
+    (oa, sa), (os_, ss) = runs[1], runs[0]
+    assert "Queued: t=0s (initial state)" in sa and "Async output finished." in sa
+    assert "Queued:" not in ss and "Saved:" in ss
+    ha, hs = _saves(oa, "H"), _saves(os_, "H")
+    assert ha == hs and len(ha) >= 4
+    assert _saves(oa, "Vx") == [f.replace("H_", "Vx_") for f in ha]
+    assert _saves(oa, "Vy") == [f.replace("H_", "Vy_") for f in ha]
+    assert _saves(os_, "Vx") == []
+    for f in ha:
+        hdr_a, da = _read_asc(os.path.join(oa, f))
+        hdr_s, ds = _read_asc(os.path.join(os_, f))
+        assert hdr_a[2] == "xllcorner     1234.500000\n" and hdr_a[3] == "yllcorner     987.250000\n"
+        assert hdr_a[5] == "NODATA_value  -9999\n"
+        assert hdr_s[2] == "xllcorner     1234.5\n" and hdr_s[3] == "yllcorner     987.2\n"
+        assert np.array_equal(da, ds), f
+    _, vx_last = _read_asc(os.path.join(oa, ha[-1].replace("H_", "Vx_")))
+    assert np.count_nonzero(vx_last) > 50
+    for f in ("H_final.asc", "Vx_final.asc", "Vy_final.asc"):
+        with open(os.path.join(oa, f)) as a, open(os.path.join(os_, f)) as b:
+            assert a.read() == b.read(), f
+    sub = tmp_path / "nc"
+    sub.mkdir()
+    par = _write_small_case(sub, "saveint 60\noutput_format 1\n")
+    r = subprocess.run([EXE, "--path", path, "--quiet", par], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = str(sub / "case" / "results")
+    assert _saves(out, "H") == [] and os.path.exists(os.path.join(out, "H_final.asc"))

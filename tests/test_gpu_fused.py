@@ -229,3 +229,27 @@ def test_read_cells_equals_full_read(gpu, prec):
         assert np.array_equal(got[inside], H[idx[inside]])
         assert np.all(np.isnan(got[~inside]))
     assert d.read_cells(np.empty(0, dtype=np.int64)).size == 0
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_snapshot_equals_read_and_velocity(gpu, prec):
+    """The device output snapshot (af_batch_snapshot) enqueued behind steps
+    that have not run yet gives H equal to a full read, and Vx, Vy equal to
+    the oracle's velocity (kernels.cpp:293-351) of that state; at both state
+    parities, and without velocity."""
+    scn = _small(81, 150, 47, sim_time=400.0)
+    d = Domain(scn, precision=prec, tuning=make_tuning(tile_rows=8))
+    nod = scn.nodata
+    for steps in (30, 11):                     # 30, then 41 steps: both parities
+        d.advance(steps, graph=False)          # enqueued; the snapshot follows on the stream
+        H, Vx, Vy = d.snapshot(velocity=True)
+        assert np.array_equal(H, d.read("H"))
+        Qx, Qy = d.read("Qx"), d.read("Qy")
+        vx, vy = oracle.velocity(H.ravel(), scn.dem.ravel().astype(np.float64), Qx, Qy,
+                                 scn.nx, scn.ny, scn.dx, nod)
+        print(f"{prec} after {d.info()['steps']} steps: moving cells "
+              f"{int(np.count_nonzero(vx))}, {int(np.count_nonzero(vy))}", flush=True)
+        assert np.count_nonzero(vx) > 1000 and np.count_nonzero(vy) > 1000
+        assert np.array_equal(Vx.ravel(), vx) and np.array_equal(Vy.ravel(), vy)
+    (H2,) = d.snapshot(velocity=False)
+    assert np.array_equal(H2, H)
