@@ -332,10 +332,23 @@ def _full_run(scn, precision, label):
 
 def bench_accuracy(args):
     """FP32 and FP64 against FP64 references.  The 4096^2 valley case to
-    --acc-time against the FP64 CPU oracle; the valley case to its full
-    7200 s and the 16384^2 roofline case to --acc-time at FP32 against the
-    FP64 GPU path (itself pinned to the oracle on the smaller runs)."""
+    --acc-time against the FP64 CPU oracle, and to its full 7200 s at FP32
+    against the FP64 GPU path (itself pinned to the oracle on the smaller
+    runs); the 16384^2 roofline case to --roof-time (its full 3600 s by
+    default) at FP32 against the FP64 GPU path."""
     import torch
+    cases = {c.strip() for c in args.acc_cases.split(",") if c.strip()}
+    lines = []
+    if "valley" in cases:
+        lines += _accuracy_valley(args)
+    if "roofline" in cases and args.size > 0:
+        lines += _accuracy_roofline(args)
+    torch.cuda.synchronize()
+    print(json.dumps({"metric": "FP32/FP64 error against FP64 reference paths",
+                      "results": lines}), flush=True)
+
+
+def _accuracy_valley(args):
     from adaptflood_amd import generators
     from oracle import oracle
     lines = []
@@ -374,24 +387,25 @@ def bench_accuracy(args):
               "gpu_wall_s": round(w32, 3), "ref_wall_s": round(w64, 3)})
     print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
     lines.append(e)
-    if args.size > 0:
-        t = time.perf_counter()
-        big = generators.roofline_case(n=args.size)
-        big.sim_time = args.acc_time
-        print(f"[acc] roofline {big.nx}x{big.ny} generated in {time.perf_counter() - t:.1f}s",
-              file=sys.stderr, flush=True)
-        H64, i64, w64 = _full_run(big, "fp64", "roofline")
-        H32, i32, w32 = _full_run(big, "fp32", "roofline")
-        e = _errors(H32, H64, big.dx * big.dx)
-        e.update({"case": f"roofline {big.nx}x{big.ny}, {big.sim_time:.0f} s",
-                  "precision": "fp32", "against": "FP64 GPU path", "steps": i32["steps"],
-                  "steps_ref": i64["steps"], "gpu_wall_s": round(w32, 3),
-                  "ref_wall_s": round(w64, 3)})
-        print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
-        lines.append(e)
-    torch.cuda.synchronize()
-    print(json.dumps({"metric": "FP32/FP64 error against FP64 reference paths",
-                      "results": lines}), flush=True)
+    return lines
+
+
+def _accuracy_roofline(args):
+    from adaptflood_amd import generators
+    t = time.perf_counter()
+    big = generators.roofline_case(n=args.size)
+    big.sim_time = args.roof_time
+    print(f"[acc] roofline {big.nx}x{big.ny} generated in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    H64, i64, w64 = _full_run(big, "fp64", "roofline")
+    H32, i32, w32 = _full_run(big, "fp32", "roofline")
+    e = _errors(H32, H64, big.dx * big.dx)
+    e.update({"case": f"roofline {big.nx}x{big.ny}, {big.sim_time:.0f} s",
+              "precision": "fp32", "against": "FP64 GPU path", "steps": i32["steps"],
+              "steps_ref": i64["steps"], "gpu_wall_s": round(w32, 3),
+              "ref_wall_s": round(w64, 3)})
+    print(f"[acc] {json.dumps(e)}", file=sys.stderr, flush=True)
+    return [e]
 
 
 def bench_divergence(args):
@@ -503,7 +517,11 @@ def main():
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
     ap.add_argument("--acc-time", type=float, default=600.0,
-                    help="accuracy: simulated seconds of the oracle and roofline comparisons")
+                    help="accuracy: simulated seconds of the valley comparison with the oracle")
+    ap.add_argument("--roof-time", type=float, default=3600.0,
+                    help="accuracy: simulated seconds of the roofline-case FP32/FP64 comparison")
+    ap.add_argument("--acc-cases", default="valley,roofline",
+                    help="accuracy: comma-separated cases to run (valley, roofline)")
     ap.add_argument("--members", type=int, default=512, help="ensemble / served scenarios")
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
                     help="ensemble / served state precision")
