@@ -418,26 +418,31 @@ def bench_divergence(args):
     from adaptflood_amd.domain import Domain, make_tuning
     from oracle import oracle
     out = {"metric": "max |dH| between runs at equal step counts", "series": []}
+    cases_on = {c.strip() for c in args.div_cases.split(",") if c.strip()}
 
     def record(name, rows):
         out["series"].append({"case": name, "points": rows})
-        print(f"[div] {name}: " + ", ".join(f"{k}:{v:.2e}" for k, v, _ in rows),
+        print(f"[div] {name}: " + ", ".join(f"{k}:{v:.2e}" for k, v, *_ in rows),
               file=sys.stderr, flush=True)
 
-    valley = generators.valley_case(n=args.valley_size)
-    r = oracle.OracleRun(valley, dt_log_cap=1, mass_log_cap=1)
-    d = Domain(valley, precision="fp64", tuning=make_tuning())
-    rows, done = [], 0
-    while done < 600:
-        r.advance(100)
-        d.advance(100)
-        done += 100
-        diff = float(np.max(np.abs(d.read("H").ravel() - r.H)))
-        rows.append((done, diff, None))
-    record(f"valley {valley.nx}^2: FP64 fused vs FP64 CPU oracle", rows)
-    d.close()
-    del r
-    cases = [("valley", valley, 400, 40), ("roofline", generators.roofline_case(n=8192), 1200, 200)]
+    cases = []
+    if "valley" in cases_on:
+        valley = generators.valley_case(n=args.valley_size)
+        r = oracle.OracleRun(valley, dt_log_cap=1, mass_log_cap=1)
+        d = Domain(valley, precision="fp64", tuning=make_tuning())
+        rows, done = [], 0
+        while done < 600:
+            r.advance(100)
+            d.advance(100)
+            done += 100
+            diff = float(np.max(np.abs(d.read("H").ravel() - r.H)))
+            rows.append((done, diff, None, None))
+        record(f"valley {valley.nx}^2: FP64 fused vs FP64 CPU oracle", rows)
+        d.close()
+        del r
+        cases.append(("valley", valley, 400, 40))
+    if "roofline" in cases_on:
+        cases.append(("roofline", generators.roofline_case(n=8192), 1200, 200))
     for name, scn, steps, every in cases:
         a = Domain(scn, precision="fp64", tuning=make_tuning())
         b = Domain(scn, precision="fp32", tuning=make_tuning())
@@ -448,8 +453,11 @@ def bench_divergence(args):
             done += every
             Ha, Hb = a.read("H"), b.read("H")
             rms = float(np.sqrt(np.mean((Ha - Hb) ** 2)))
-            rows.append((done, float(np.max(np.abs(Ha - Hb))), rms))
-        record(f"{name} {scn.nx}^2: FP32 vs FP64 fused", rows)
+            va, vb = a.volume()[0], b.volume()[0]
+            rows.append((done, float(np.max(np.abs(Ha - Hb))), rms,
+                         abs(vb - va) / va if va else 0.0))
+        record(f"{name} {scn.nx}^2: FP32 vs FP64 fused (step, max, RMS, rel. volume diff)",
+               rows)
         a.close()
         b.close()
     torch.cuda.synchronize()
@@ -520,6 +528,8 @@ def main():
                     help="accuracy: simulated seconds of the valley comparison with the oracle")
     ap.add_argument("--roof-time", type=float, default=3600.0,
                     help="accuracy: simulated seconds of the roofline-case FP32/FP64 comparison")
+    ap.add_argument("--div-cases", default="valley,roofline",
+                    help="divergence: comma-separated cases to run (valley, roofline)")
     ap.add_argument("--acc-cases", default="valley,roofline",
                     help="accuracy: comma-separated cases to run (valley, roofline)")
     ap.add_argument("--members", type=int, default=512, help="ensemble / served scenarios")
