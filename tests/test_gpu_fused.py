@@ -253,3 +253,61 @@ def test_snapshot_equals_read_and_velocity(gpu, prec):
         assert np.array_equal(Vx.ravel(), vx) and np.array_equal(Vy.ravel(), vy)
     (H2,) = d.snapshot(velocity=False)
     assert np.array_equal(H2, H)
+
+
+def _degenerate(kind):
+    """Inputs at the edges of what the step accepts."""
+    from adaptflood_amd.scenario import PointBC, Scenario
+    from adaptflood_amd._lib import AF_BC_FREE, AF_BC_HFIX, AF_BC_QFIX, AF_BC_QVAR
+    rng = np.random.default_rng(5)
+    if kind == "2x2":                  # the smallest grid, a source and all edges free
+        return Scenario(dem=np.array([[0.3, 0.1], [0.2, 0.0]]), dx=5.0, sim_time=200.0,
+                        west_free=True, east_free=True, north_free=True, south_free=True,
+                        points=[PointBC(0, 0, AF_BC_QFIX, 0.4)])
+    if kind in ("2x300", "300x2"):     # one strip-edge column pair / two rows of strips
+        ny, nx = (300, 2) if kind == "2x300" else (2, 300)
+        dem = rng.uniform(0, 1, (ny, nx))
+        return Scenario(dem=dem, dx=4.0, sim_time=120.0, east_free=True, south_free=True,
+                        points=[PointBC(1, 1, AF_BC_QFIX, 0.3), PointBC(0, ny - 1, AF_BC_HFIX, 0.5)],
+                        rain_times=np.array([0.0, 100.0]), rain_rates=np.array([2e-5, 0.0]))
+    if kind == "all_nodata":           # no valid cell: dt stays InitTstep, sources still add
+        dem = np.full((9, 70), -9999.0)
+        return Scenario(dem=dem, dx=10.0, sim_time=95.0, west_free=True, north_free=True,
+                        points=[PointBC(3, 4, AF_BC_QFIX, 0.2),
+                                PointBC(60, 2, AF_BC_QVAR, 0.0, np.array([0.0, 50.0]),
+                                        np.array([0.0, 1.0])),
+                                PointBC(10, 8, AF_BC_HFIX, 0.7), PointBC(11, 8, AF_BC_FREE)],
+                        rain_times=np.array([0.0]), rain_rates=np.array([1e-4]))
+    if kind == "still_water":          # no points, no rain: water at rest in a basin
+        dem = np.full((40, 90), 2.0)
+        dem[5:35, 5:85] = rng.integers(0, 64, (30, 80)) / 64.0
+        return Scenario(dem=dem, dx=2.0, sim_time=300.0, h0=np.where(dem < 2.0, 1.5 - dem, 0.0))
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["2x2", "2x300", "300x2", "all_nodata", "still_water"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_degenerate_inputs(gpu, kind, prec):
+    """The smallest grid, one- and two-strip shapes, an all-NODATA grid (the
+    H max is empty, so dt stays InitTstep while sources and overrides still
+    act) and still water with no points and no rain: FP64 bitwise equal to the
+    oracle, FP32 within its bar."""
+    scn = _degenerate(kind)
+    ref = oracle.run(scn)
+    d = Domain(scn, precision=prec)
+    d.run(check_every=16, graph=True)
+    H = d.read("H")
+    err = float(np.max(np.abs(H - ref["H"])))
+    print(f"{kind} {prec}: steps={d.info()['steps']} (oracle {ref['steps']}) max|dH|={err:.2e}",
+          flush=True)
+    if kind == "all_nodata":
+        assert np.all(ref["dt"] == scn.init_tstep) and ref["H"][4, 3] > 0
+    if kind == "still_water":
+        assert np.array_equal(ref["H"], scn.h0)
+    if prec == "fp64":
+        assert d.info()["steps"] == ref["steps"]
+        np.testing.assert_array_equal(H, ref["H"])
+        np.testing.assert_array_equal(d.read("Qx"), ref["Qx"])
+        np.testing.assert_array_equal(d.read("Qy"), ref["Qy"])
+    else:
+        assert err <= FP32_TOL
