@@ -50,6 +50,7 @@ class Tuning(C.Structure):
     _fields_ = [
         ("tile_rows", C.c_int32), ("waves_per_block", C.c_int32), ("graph_steps", C.c_int32),
         ("dt_log_cap", C.c_int32), ("report_cap", C.c_int32), ("variant", C.c_int32),
+        ("tile_order", C.c_int32),
     ]
 
 

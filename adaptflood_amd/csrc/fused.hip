@@ -65,6 +65,7 @@ struct af_domain {
     int precision = AF_FP32;
     int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36, variant = 0;
     int R_fixed = 0;             // tile rows asked for (0: chosen per scenario)
+    int tile_order = 0;          // af_tuning.tile_order
     long long dt_log_cap = 0;
     int report_cap = 64;
     char* arena = nullptr;
@@ -278,6 +279,9 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
         AF_REQUIRE(tun->variant >= 0 && tun->variant <= 3, AF_E_INVALID,
                    "variant %d (0 = default, 1..3)", tun->variant);
         if (tun->variant > 0) D->variant = tun->variant - 1;
+        AF_REQUIRE(tun->tile_order == 0 || tun->tile_order == 1, AF_E_INVALID,
+                   "tile_order %d (0 or 1)", tun->tile_order);
+        D->tile_order = tun->tile_order;
     }
     AF_REQUIRE(D->R >= 1 && D->R <= 65536, AF_E_INVALID, "tile_rows %d", D->R);
     AF_REQUIRE(D->wpb == 1 || D->wpb == 4, AF_E_INVALID, "waves_per_block %d (1 or 4)", D->wpb);
@@ -634,6 +638,22 @@ int rebuild_tiles(af_domain* D, hipStream_t st) {
     }
     AF_REQUIRE((int)tiles.size() <= D->max_tiles, AF_E_CAPACITY, "tile table full (%zu > %d)",
                tiles.size(), D->max_tiles);
+    if (D->tile_order == 1) {
+        // Largest first: workgroups are dispatched roughly in table order as
+        // slots free up, so the step's tail is made of the short tiles (the
+        // last segment of a scenario, the partial strip of a ragged width).
+        // The kernel indexes per-scenario data by (strip, segment), so the
+        // order of the table does not change any result.
+        auto cells = [&](const TileDesc& t) {
+            const HostScn& h = D->scn[t.scn];
+            const long long rows = std::min(h.R, h.ny - t.seg * h.R);
+            const long long cols = std::min(D->W, h.nx - t.strip * D->W);
+            return rows * cols;
+        };
+        std::stable_sort(tiles.begin(), tiles.end(), [&](const TileDesc& a, const TileDesc& b) {
+            return cells(a) > cells(b);
+        });
+    }
     if (!tiles.empty())
         AF_HIP(hipMemcpyAsync(D->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
                               hipMemcpyHostToDevice, st));

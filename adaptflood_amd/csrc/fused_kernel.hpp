@@ -102,6 +102,9 @@ using Vec = T __attribute__((ext_vector_type(V)));
 // flat_*) instructions and uniform loads can go through the scalar unit.
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
 template <typename T>
 __device__ __forceinline__ gptr<T> glob(const void* p) {
     return (gptr<T>)(const_cast<void*>(p));
@@ -173,7 +176,7 @@ struct TileCtx {
     gptr<const T> z, Ho, Qxo, Qyo;
     gptr<T> Hn, Qxn, Qyn;
     long long P;
-    int nx, ny, i0, j1, c0, rc, rf, lane;
+    int nx, ny, i0, j0, j1, c0, rc, rf, lane;
     int hcol_c, hcol_f;
     bool hcell, hface;
     T nod, sc, dxT, gT, dtdA, rainT;
@@ -181,15 +184,28 @@ struct TileCtx {
     double t, dt, dA;
     FluxConstsF64 kd;
     FluxConstsF32 kf;
+    lptr<int> lrc;   // this wave's point batches in LDS: sources first, then overrides;
+    lptr<T> lval;    // packed row/column and the value applied
 };
 
-// Per-tile values carried from row to row.  The point-list cursors live in
-// scalar registers: they are uniform, and a vector-memory value here would
-// make every row wait for all loads in flight.
+// Per-tile values carried from row to row.
+//
+// Point lists.  At the tile's start, the first kPointBatch sources (and,
+// separately, overrides) of the tile are evaluated across the lanes, one
+// entry per lane and round, and their packed row and column,
+// (row - j0) << 9 | (column - i0), and the value they apply go to the wave's
+// LDS slice.  The row loop applies them from LDS in list order, and takes any
+// entries past the batch one at a time with uniform loads.  The cursors are
+// uniform and live in scalar registers; the row of the next batched entry
+// comes from LDS, so no row waits on the vector loads in flight to decide
+// whether a point falls on it.
+constexpr int kPointBatch = 128;
+
 template <typename T, int V>
 struct TileState {
     T qyN[V];
-    int sp, se, src_next_j, op, oe, ovr_next_j;
+    int sp, se, sk, sn, src_next_j;   // sources: first, end, next (from sp), batched; next row
+    int op, oe, ok, on, ovr_next_j;   // overrides: likewise
     T hmax;
     double vol_s, vol_c;
     unsigned long long wet;
@@ -398,66 +414,146 @@ struct Stepper {
         if (two) store<EDGE>(x, o1, j + 1);
     }
 
-    // One QFIX/QVAR source: v at the step's start time, ps = v*dx*dt/dA added
-    // with a clamp (main.cpp:246-260).
-    static __device__ __forceinline__ void apply_source(const ScnDev& S,
-                                                                  const TileCtx<T>& x,
-                                                                  TileState<T, V>& st, T (&hn)[V]) {
-        const int sp = st.sp;
-        const int col = glob<const int>(S.src_i)[sp];
-        const int pt = glob<const int>(S.src_pt)[sp];
+    // Value of source entry k (the list is sorted per tile): v at the step's
+    // start time, then ps = v*dx*dt/dA (main.cpp:246-253) as the depth added.
+    static __device__ __forceinline__ T source_depth(const ScnDev& S, const TileCtx<T>& x, int k) {
+        const int pt = glob<const int>(S.src_pt)[k];
         gptr<const double> pt_value = glob<const double>(S.pt_value);
-        gptr<const int> pt_off = glob<const int>(S.pt_off);
         double val;
-        if (glob<const int>(S.pt_type)[pt] == 4) val = pt_value[pt];
-        else val = interp_series(glob<const double>(S.series_t) + pt_off[pt],
-                                 glob<const double>(S.series_v) + pt_off[pt],
-                                 glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+        if (glob<const int>(S.pt_type)[pt] == 4) {
+            val = pt_value[pt];
+        } else {
+            const int off = glob<const int>(S.pt_off)[pt];
+            val = interp_series(glob<const double>(S.series_t) + off,
+                                glob<const double>(S.series_v) + off,
+                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+        }
         const double ps = val * S.dx * x.dt / x.dA;
+        return (T)ps;
+    }
+
+    // Value of override entry k: 0 (FREE), the fixed depth (HFIX) or the
+    // depth series at the step's start time (HVAR); kernels.cpp:266-273,
+    // main.cpp:275-279.
+    static __device__ __forceinline__ T override_depth(const ScnDev& S, const TileCtx<T>& x,
+                                                       int k) {
+        const int pt = glob<const int>(S.ovr_pt)[k];
+        const int ty = glob<const int>(S.pt_type)[pt];
+        gptr<const double> pt_value = glob<const double>(S.pt_value);
+        double val = 0.0;
+        if (ty == 2) {
+            val = pt_value[pt];
+        } else if (ty == 3) {
+            const int off = glob<const int>(S.pt_off)[pt];
+            val = interp_series(glob<const double>(S.series_t) + off,
+                                glob<const double>(S.series_v) + off,
+                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+        }
+        return (T)val;
+    }
+
+    // At the tile's start: the first kPointBatch sources and overrides of the
+    // tile, evaluated across the lanes (entry base + 64 r + lane in round r)
+    // into the wave's LDS slice.  Entries past the batch are taken one at a
+    // time in the row loop.
+    static __device__ __forceinline__ void load_points(const ScnDev& S, const TileCtx<T>& x,
+                                                       TileState<T, V>& st) {
+        st.sn = min(kPointBatch, st.se - st.sp);
+        st.on = min(kPointBatch, st.oe - st.op);
+        for (int r = 0; r < st.sn; r += 64) {
+            const int e = r + x.lane;
+            if (e < st.sn) {
+                const int k = st.sp + e;
+                const int j = glob<const int>(S.src_j)[k];
+                const int col = glob<const int>(S.src_i)[k];
+                x.lrc[e] = ((j - x.j0) << 9) | (col - x.i0);
+                x.lval[e] = source_depth(S, x, k);
+            }
+        }
+        for (int r = 0; r < st.on; r += 64) {
+            const int e = r + x.lane;
+            if (e < st.on) {
+                const int k = st.op + e;
+                const int j = glob<const int>(S.ovr_j)[k];
+                const int col = glob<const int>(S.ovr_i)[k];
+                x.lrc[kPointBatch + e] = ((j - x.j0) << 9) | (col - x.i0);
+                x.lval[kPointBatch + e] = override_depth(S, x, k);
+            }
+        }
+        st.sk = 0;
+        st.ok = 0;
+        st.src_next_j = (st.sn > 0) ? x.j0 + (__builtin_amdgcn_readfirstlane(x.lrc[0]) >> 9)
+                                    : INT_MAX;
+        st.ovr_next_j = (st.on > 0)
+                            ? x.j0 + (__builtin_amdgcn_readfirstlane(x.lrc[kPointBatch]) >> 9)
+                            : INT_MAX;
+    }
+
+    // The next source, on row src_next_j: its depth is added to its cell
+    // with a clamp (main.cpp:256-260), in list order, which is file order on
+    // a shared cell.
+    static __device__ __forceinline__ void apply_source(const ScnDev& S, const TileCtx<T>& x,
+                                                        TileState<T, V>& st, T (&hn)[V]) {
+        const int k = st.sk;
+        int col;
+        T dh;
+        if (k < st.sn) {
+            col = x.i0 + (__builtin_amdgcn_readfirstlane(x.lrc[k]) & 511);
+            dh = x.lval[k];
+        } else {  // past the batch: one entry at a time, uniform loads
+            col = __builtin_amdgcn_readfirstlane(glob<const int>(S.src_i)[st.sp + k]);
+            dh = source_depth(S, x, st.sp + k);
+        }
 #pragma unroll
         for (int v = 0; v < V; v++) {
             if (col == x.c0 + v) {
-                T h = hn[v] + (T)ps;
+                T h = hn[v] + dh;
                 if (h < T(0)) h = T(0);
                 hn[v] = h;
             }
         }
-        st.sp = sp + 1;
-        st.src_next_j = __builtin_amdgcn_readfirstlane(
-            (sp + 1 < st.se) ? glob<const int>(S.src_j)[sp + 1] : INT_MAX);
+        const int n = k + 1;
+        st.sk = n;
+        if (n < st.sn)
+            st.src_next_j = x.j0 + (__builtin_amdgcn_readfirstlane(x.lrc[n]) >> 9);
+        else if (st.sp + n < st.se)
+            st.src_next_j = __builtin_amdgcn_readfirstlane(glob<const int>(S.src_j)[st.sp + n]);
+        else
+            st.src_next_j = INT_MAX;
     }
 
-    // One FREE/HFIX/HVAR point: H = 0, the fixed depth, or the depth series
-    // at the step's start time (kernels.cpp:266-273, main.cpp:275-279).
-    static __device__ __forceinline__ void apply_override(const ScnDev& S,
-                                                                    const TileCtx<T>& x,
-                                                                    TileState<T, V>& st,
-                                                                    T (&hn)[V]) {
-        const int op = st.op;
-        const int col = glob<const int>(S.ovr_i)[op];
-        const int pt = glob<const int>(S.ovr_pt)[op];
-        const int ty = glob<const int>(S.pt_type)[pt];
-        gptr<const double> pt_value = glob<const double>(S.pt_value);
-        gptr<const int> pt_off = glob<const int>(S.pt_off);
-        double val = 0.0;
-        if (ty == 2) val = pt_value[pt];
-        else if (ty == 3)
-            val = interp_series(glob<const double>(S.series_t) + pt_off[pt],
-                                glob<const double>(S.series_v) + pt_off[pt],
-                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
+    // The next FREE/HFIX/HVAR point: the last in file order on a cell wins.
+    static __device__ __forceinline__ void apply_override(const ScnDev& S, const TileCtx<T>& x,
+                                                          TileState<T, V>& st, T (&hn)[V]) {
+        const int k = st.ok;
+        int col;
+        T val;
+        if (k < st.on) {
+            col = x.i0 + (__builtin_amdgcn_readfirstlane(x.lrc[kPointBatch + k]) & 511);
+            val = x.lval[kPointBatch + k];
+        } else {
+            col = __builtin_amdgcn_readfirstlane(glob<const int>(S.ovr_i)[st.op + k]);
+            val = override_depth(S, x, st.op + k);
+        }
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            if (col == x.c0 + v) hn[v] = (T)val;
+            if (col == x.c0 + v) hn[v] = val;
         }
-        st.op = op + 1;
-        st.ovr_next_j = __builtin_amdgcn_readfirstlane(
-            (op + 1 < st.oe) ? glob<const int>(S.ovr_j)[op + 1] : INT_MAX);
+        const int n = k + 1;
+        st.ok = n;
+        if (n < st.on)
+            st.ovr_next_j = x.j0 + (__builtin_amdgcn_readfirstlane(x.lrc[kPointBatch + n]) >> 9);
+        else if (st.op + n < st.oe)
+            st.ovr_next_j = __builtin_amdgcn_readfirstlane(glob<const int>(S.ovr_j)[st.op + n]);
+        else
+            st.ovr_next_j = INT_MAX;
     }
 
     // EDGE = false: the strip touches neither the west nor the east edge and
     // every lane's columns are cells, so no per-face edge test is needed.
     template <bool EDGE>
-    static __device__ __forceinline__ void run(const ScnDev& S, const TileDesc td, int kmod6, int lane) {
+    static __device__ __forceinline__ void run(const ScnDev& S, const TileDesc td, int kmod6, int lane,
+                                               lptr<int> lrc, lptr<T> lval) {
         const int cur = kmod6 & 1, nxt = cur ^ 1;
         const int hs_cur = kmod6 % 3, hs_nxt = (kmod6 + 1) % 3, hs_rst = (kmod6 + 2) % 3;
         gptr<Ctl> ctl = glob<Ctl>(S.ctl);
@@ -483,9 +579,12 @@ struct Stepper {
         x.P = S.P;
         x.i0 = td.strip * W;
         const int j0 = td.seg * S.tile_rows;
+        x.j0 = j0;
         x.j1 = min(j0 + S.tile_rows, x.ny);
         x.c0 = x.i0 + lane * V;
         x.lane = lane;
+        x.lrc = lrc;
+        x.lval = lval;
         x.nod = (T)S.nodata_t;
         x.z = glob<const T>(S.z);
         x.Ho = glob<const T>(S.H[s_in]);
@@ -571,12 +670,12 @@ struct Stepper {
         gptr<const int> ovr_begin = glob<const int>(S.ovr_begin);
         st.sp = __builtin_amdgcn_readfirstlane(src_begin[tile_local]);
         st.se = __builtin_amdgcn_readfirstlane(src_begin[tile_local + 1]);
-        st.src_next_j = __builtin_amdgcn_readfirstlane(
-            (st.sp < st.se) ? glob<const int>(S.src_j)[st.sp] : INT_MAX);
         st.op = __builtin_amdgcn_readfirstlane(ovr_begin[tile_local]);
         st.oe = __builtin_amdgcn_readfirstlane(ovr_begin[tile_local + 1]);
-        st.ovr_next_j = __builtin_amdgcn_readfirstlane(
-            (st.op < st.oe) ? glob<const int>(S.ovr_j)[st.op] : INT_MAX);
+        st.sn = st.on = st.sk = st.ok = 0;
+        st.src_next_j = INT_MAX;
+        st.ovr_next_j = INT_MAX;
+        if (st.sp < st.se || st.op < st.oe) load_points(S, x, st);
         st.hmax = F64 ? T(-DBL_MAX) : T(-FLT_MAX);
         st.vol_s = 0.0;
         st.vol_c = 0.0;
@@ -736,10 +835,15 @@ k_step(const ScnDev* __restrict__ scns, const TileDesc* __restrict__ tiles, int 
     const TileDesc td = tiles[tile];
     const ScnDev& S = scns[td.scn];
     constexpr int W = 64 * V;
+    // per wave: the batched source and override entries of the tile
+    __shared__ int l_rc[WPB][2 * kPointBatch];
+    __shared__ T l_val[WPB][2 * kPointBatch];
+    const lptr<int> lrc = (lptr<int>)&l_rc[wv][0];
+    const lptr<T> lval = (lptr<T>)&l_val[wv][0];
     if (td.strip > 0 && (td.strip + 1) * W < S.nx)
-        Stepper<T, V, F64, PF>::template run<false>(S, td, kmod6, lane);
+        Stepper<T, V, F64, PF>::template run<false>(S, td, kmod6, lane, lrc, lval);
     else
-        Stepper<T, V, F64, PF>::template run<true>(S, td, kmod6, lane);
+        Stepper<T, V, F64, PF>::template run<true>(S, td, kmod6, lane, lrc, lval);
 }
 
 }  // namespace af

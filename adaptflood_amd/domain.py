@@ -27,10 +27,11 @@ def _stream(torch, stream=None):
 
 
 def make_tuning(tile_rows=0, waves_per_block=0, graph_steps=0, dt_log_cap=0, report_cap=0,
-                variant=0):
+                variant=0, tile_order=0):
     t = Tuning()
     t.tile_rows, t.waves_per_block, t.graph_steps = tile_rows, waves_per_block, graph_steps
     t.dt_log_cap, t.report_cap, t.variant = dt_log_cap, report_cap, variant
+    t.tile_order = tile_order
     return t
 
 

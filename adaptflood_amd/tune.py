@@ -1,7 +1,7 @@
 """Autotuner for the fused step's tile shape and kernel variant.
 
 For a workload (one scenario or a ragged batch), every candidate tuning
-(tile rows x kernel variant x waves per block) is built, warmed up and timed
+(tile rows x kernel variant x waves per block x tile order) is built, warmed up and timed
 over eager launches of the step kernel with HIP events on the launch stream;
 the fastest is returned with the whole table.  Results can be kept in a JSON
 cache keyed by the workload's shape, so a serving process tunes each shape
@@ -20,6 +20,7 @@ from .domain import Batch, PRECISIONS, _torch, make_tuning
 DEFAULT_ROWS = (8, 16, 32, 64, 128)
 DEFAULT_VARIANTS = (1, 2, 3)
 DEFAULT_WPB = (4,)
+DEFAULT_ORDERS = (0, 1)
 
 
 def signature(scenarios, precision):
@@ -49,8 +50,10 @@ def _time_candidate(torch, scenarios, precision, tuning, warmup, launches):
 
 
 def autotune(scenarios, precision="fp32", rows=DEFAULT_ROWS, variants=DEFAULT_VARIANTS,
-             waves_per_block=DEFAULT_WPB, warmup=3, launches=20, cache_path=None, log=None):
-    """Returns (best af_tuning, table of (ms per step, rows, variant, wpb)).
+             waves_per_block=DEFAULT_WPB, orders=DEFAULT_ORDERS, warmup=3, launches=20,
+             cache_path=None, log=None):
+    """Returns (best af_tuning, table of (ms per step, rows, variant, wpb,
+    tile order)).
 
     `scenarios` is a list (a single domain is a list of one).  With
     cache_path, a stored result for the same signature is returned without
@@ -65,19 +68,20 @@ def autotune(scenarios, precision="fp32", rows=DEFAULT_ROWS, variants=DEFAULT_VA
         if key in cache:
             c = cache[key]
             return (make_tuning(tile_rows=c["rows"], waves_per_block=c["wpb"],
-                                variant=c["variant"]), c["table"])
+                                variant=c["variant"], tile_order=c.get("order", 0)),
+                    c["table"])
     torch = _torch()
     table = []
-    for r, v, w in itertools.product(rows, variants, waves_per_block):
-        tun = make_tuning(tile_rows=r, waves_per_block=w, variant=v)
+    for r, v, w, o in itertools.product(rows, variants, waves_per_block, orders):
+        tun = make_tuning(tile_rows=r, waves_per_block=w, variant=v, tile_order=o)
         ms = _time_candidate(torch, scenarios, precision, tun, warmup, launches)
-        table.append((ms, r, v, w))
+        table.append((ms, r, v, w, o))
         if log:
-            log(f"rows={r} variant={v} wpb={w}: {ms:.4f} ms/step")
+            log(f"rows={r} variant={v} wpb={w} order={o}: {ms:.4f} ms/step")
     table.sort()
-    ms, r, v, w = table[0]
+    ms, r, v, w, o = table[0]
     if cache_path:
-        cache[key] = {"rows": r, "variant": v, "wpb": w, "ms": ms, "table": table}
+        cache[key] = {"rows": r, "variant": v, "wpb": w, "order": o, "ms": ms, "table": table}
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1)
-    return make_tuning(tile_rows=r, waves_per_block=w, variant=v), table
+    return make_tuning(tile_rows=r, waves_per_block=w, variant=v, tile_order=o), table

@@ -158,9 +158,10 @@ def bench_ensemble(args):
         best, table = autotune(scns, prec, rows=(16, 32, 64, 128), launches=10,
                                log=lambda m: print(f"[tune] {m}", file=sys.stderr, flush=True))
         tun = make_tuning(tile_rows=best.tile_rows, waves_per_block=best.waves_per_block,
-                          variant=best.variant, graph_steps=36)
+                          variant=best.variant, graph_steps=36, tile_order=best.tile_order)
         tuned = {"tile_rows": best.tile_rows, "variant": best.variant,
-                 "waves_per_block": best.waves_per_block, "tuning_s": round(time.perf_counter() - t, 1)}
+                 "waves_per_block": best.waves_per_block, "tile_order": best.tile_order,
+                 "tuning_s": round(time.perf_counter() - t, 1)}
         print(f"[bench] autotuned: {tuned}", file=sys.stderr, flush=True)
     b = Batch(scns, precision=prec, tuning=tun)
     b.advance(args.warmup, graph=False)

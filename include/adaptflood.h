@@ -188,6 +188,9 @@ typedef struct af_tuning {
     int32_t report_cap;      /* device mass-report records kept */
     int32_t variant;         /* 0 = library default; 1..3 = step-kernel variant: 1 one-row
                               * loop, 2 and 3 two-row bursts at lower target occupancy */
+    int32_t tile_order;      /* order of the launch's tile table: 0 = by scenario, segment,
+                              * strip; 1 = largest tiles (rows x columns) first, so that the
+                              * last workgroups dispatched are the short ones */
 } af_tuning;
 
 /* Device bytes the domain needs for `desc` at `precision`. */

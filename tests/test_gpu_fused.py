@@ -147,13 +147,15 @@ def test_graph_equals_eager(gpu):
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
 def test_tiling_invariance(gpu, prec):
-    """Results do not depend on the tile shape or the kernel variant (bitwise);
-    a wide grid (nx = 2100) takes the 1 KiB row pitch."""
+    """Results do not depend on the tile shape, the kernel variant or the tile
+    order (bitwise); a wide grid (nx = 2100) takes the 1 KiB row pitch."""
     scn = _small(31, 2100, 141, sim_time=60.0)
     outs = []
-    for rows, wpb, var in ((4, 1, 1), (32, 4, 0), (128, 4, 2), (61, 4, 3), (33, 1, 2)):
+    for rows, wpb, var, order in ((4, 1, 1, 0), (32, 4, 0, 0), (128, 4, 2, 0), (61, 4, 3, 1),
+                                  (33, 1, 2, 1)):
         d = Domain(scn, precision=prec,
-                   tuning=make_tuning(tile_rows=rows, waves_per_block=wpb, variant=var))
+                   tuning=make_tuning(tile_rows=rows, waves_per_block=wpb, variant=var,
+                                      tile_order=order))
         d.run(check_every=25, graph=False)
         outs.append((d.read("H"), d.read("Qx"), d.read("Qy")))
     for o in outs[1:]:
@@ -162,10 +164,11 @@ def test_tiling_invariance(gpu, prec):
 
 
 def test_batch_equals_solo(gpu):
-    """Each scenario of a ragged batch gives bitwise the result it gives alone."""
+    """Each scenario of a ragged batch, its tiles in largest-first order, gives
+    bitwise the result it gives alone."""
     scns = [_small(40 + k, nx, ny, sim_time=100.0 + 30 * k)
             for k, (nx, ny) in enumerate([(50, 40), (130, 21), (17, 90), (260, 64)])]
-    b = Batch(scns, precision="fp32", tuning=make_tuning(tile_rows=16))
+    b = Batch(scns, precision="fp32", tuning=make_tuning(tile_rows=16, tile_order=1))
     b.run(check_every=32, graph=True)
     for k, s in enumerate(scns):
         d = Domain(s, precision="fp32", tuning=make_tuning(tile_rows=16))
@@ -202,14 +205,16 @@ def test_autotune_picks_a_candidate_and_caches(gpu, tmp_path):
     scns = [_small(60 + k, nx, ny, sim_time=50.0) for k, (nx, ny) in
             enumerate([(300, 40), (90, 130)])]
     cache = str(tmp_path / "tune.json")
-    tun, table = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), launches=5,
-                          cache_path=cache)
-    assert len(table) == 4
-    assert (tun.tile_rows, tun.variant) in {(r, v) for _, r, v, _ in table}
+    tun, table = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), orders=(0, 1),
+                          launches=5, cache_path=cache)
+    assert len(table) == 8
+    assert (tun.tile_rows, tun.variant, tun.tile_order) in {(r, v, o) for _, r, v, _, o in table}
     assert table[0][1] == tun.tile_rows and table[0][2] == tun.variant
+    assert table[0][4] == tun.tile_order
     again, table2 = autotune(scns, "fp32", rows=(8, 32), variants=(1, 2), cache_path=cache)
-    assert (again.tile_rows, again.variant) == (tun.tile_rows, tun.variant)
-    assert len(table2) == 4
+    assert (again.tile_rows, again.variant, again.tile_order) == \
+        (tun.tile_rows, tun.variant, tun.tile_order)
+    assert len(table2) == 8
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
