@@ -316,3 +316,56 @@ def test_degenerate_inputs(gpu, kind, prec):
         np.testing.assert_array_equal(d.read("Qy"), ref["Qy"])
     else:
         assert err <= FP32_TOL
+
+
+def _dense_points_case():
+    """A 70 x 64 grid with 300 sources and 200 overrides (duplicates on shared
+    cells included), enough that one 64-row tile holds more than the 128
+    entries per list that a tile evaluates at its start."""
+    from adaptflood_amd.scenario import PointBC
+    from adaptflood_amd._lib import AF_BC_FREE, AF_BC_HFIX, AF_BC_HVAR, AF_BC_QFIX, AF_BC_QVAR
+    scn = _small(97, 70, 64, sim_time=120.0, n_src=0, n_over=0, rain=True)
+    rng = np.random.default_rng(97)
+    valid = np.argwhere(scn.dem != scn.nodata)
+    pts = []
+    for k in range(300):
+        y, x = valid[rng.integers(len(valid))] if k % 7 else valid[k % 5]   # repeats
+        if k % 3 == 0:
+            pts.append(PointBC(int(x), int(y), AF_BC_QFIX, float(rng.uniform(-0.05, 0.4))))
+        else:
+            tp = float(rng.uniform(10.0, 100.0))
+            pts.append(PointBC(int(x), int(y), AF_BC_QVAR, 0.0, np.array([0.0, tp, 2 * tp]),
+                               np.array([0.0, float(rng.uniform(0.05, 0.5)), 0.0])))
+    for k in range(200):
+        y, x = valid[rng.integers(len(valid))] if k % 9 else valid[7 + k % 3]
+        kind = k % 3
+        if kind == 0:
+            pts.append(PointBC(int(x), int(y), AF_BC_FREE))
+        elif kind == 1:
+            pts.append(PointBC(int(x), int(y), AF_BC_HFIX, float(rng.uniform(0.1, 0.6))))
+        else:
+            pts.append(PointBC(int(x), int(y), AF_BC_HVAR, 0.0, np.array([0.0, 120.0]),
+                               np.array([0.05, float(rng.uniform(0.2, 0.9))])))
+    scn.points = pts
+    return scn
+
+
+@pytest.mark.parametrize("prec,rows", [("fp64", 64), ("fp64", 16), ("fp32", 64)])
+def test_tile_with_more_points_than_the_batch(gpu, prec, rows):
+    """A tile whose point lists overflow the per-tile batch takes the rest one
+    at a time: FP64 stays bitwise equal to the oracle (file order on shared
+    cells included) and FP32 within its bar."""
+    scn = _dense_points_case()
+    ref = oracle.run(scn)
+    d = Domain(scn, precision=prec, tuning=make_tuning(tile_rows=rows))
+    d.run(check_every=20, graph=True)
+    H = d.read("H")
+    err = float(np.max(np.abs(H - ref["H"])))
+    print(f"{prec} rows={rows}: steps={d.info()['steps']} (oracle {ref['steps']}) "
+          f"max|dH|={err:.2e}", flush=True)
+    if prec == "fp64":
+        assert d.info()["steps"] == ref["steps"]
+        np.testing.assert_array_equal(H, ref["H"])
+        np.testing.assert_array_equal(d.read("Qx"), ref["Qx"])
+    else:
+        assert err <= FP32_TOL
