@@ -141,6 +141,9 @@ SIGNATURES = [
     ("af_interp_series", _D, [_dp, _dp, _I32, _D, _D]),
     ("af_write_asc", _I, [C.c_char_p, _dp, _I32, _I32, _D, _D, _D]),
     ("af_write_asc_async", _I, [C.c_char_p, _dp, _I32, _I32, _D, _D, _D, _D]),
+    ("af_nc_create", _I, [C.c_char_p, _I32, _I32, _D, _D, _D, _D, _I32, C.POINTER(_V)]),
+    ("af_nc_append", _I, [_V, _D, _dp, _dp, _dp]),
+    ("af_nc_close", _I, [_V]),
 ]
 
 _lib = None

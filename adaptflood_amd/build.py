@@ -30,7 +30,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-pthread"]
 
 HIP_SOURCES = ["compat.hip", "fused.hip"]
-CXX_SOURCES = ["io.cpp", "common.cpp"]
+CXX_SOURCES = ["io.cpp", "common.cpp", "ncwriter.cpp"]
 
 
 def _run(cmd):

@@ -17,11 +17,12 @@
 // when it names a stage file (main.cpp:95-113, 306-322).
 //
 // With async_output on (the default, types.hpp:37) the saves go through a
-// writer thread as in AsyncOutputManager (async_output.cpp): "Queued:" lines,
-// files in that writer's header format, Vx_%04d/Vy_%04d.asc beside H_%04d.asc
-// when velocity output is on, and no ASC saves for output_format 1 (NetCDF,
-// which this build does not write).  Each save is a device snapshot into one
-// of two slots, copied to pinned host memory on a side stream while the
+// writer thread as in AsyncOutputManager (async_output.cpp): "Queued:" lines;
+// with output_format 1 or 2 a record per save in output.nc (the classic
+// 64-bit-offset NetCDF format, see ncwriter.cpp); with output_format 0 or 2
+// ASC files in that writer's header format, Vx_%04d/Vy_%04d.asc beside
+// H_%04d.asc when velocity output is on.  Each save is a device snapshot into
+// one of two slots, copied to pinned host memory on a side stream while the
 // steps go on, and written by the thread.
 #include <hip/hip_runtime_api.h>
 #include <sys/stat.h>
@@ -136,8 +137,16 @@ class AsyncWriter {
         bool busy = false;
     };
 
-    AsyncWriter(const Output& out, bool velocity, bool asc)
+    AsyncWriter(const Output& out, bool velocity, bool asc, bool netcdf)
         : out_(out), velocity_(velocity), asc_(asc), n_((size_t)out.g.nx * out.g.ny) {
+        if (netcdf) {  // async_output.cpp:18-54
+            const std::string path = out.dir + "/output.nc";
+            af_ok(af_nc_create(path.c_str(), out.g.nx, out.g.ny, out.g.xll, out.g.yll,
+                               out.g.cell, out.g.nodata, velocity ? 1 : 0, &nc_),
+                  "nc_create");
+            printf("NetCDF file created: %s\n", path.c_str());
+            fflush(stdout);
+        }
         hip_ok(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
         const int fields = velocity_ ? 3 : 1;
         for (Slot& s : slots_) {
@@ -163,6 +172,7 @@ class AsyncWriter {
             if (s.copied) (void)hipEventDestroy(s.copied);
         }
         if (copy_) (void)hipStreamDestroy(copy_);
+        if (nc_) (void)af_nc_close(nc_);
     }
 
     bool velocity() const { return velocity_; }
@@ -180,8 +190,9 @@ class AsyncWriter {
         }
     }
 
-    // The slot's device buffers are (being) filled on `compute`.
-    void submit(Slot& s, int save_num, hipStream_t compute) {
+    // The slot's device buffers, the state at time t, are (being) filled on
+    // `compute`.
+    void submit(Slot& s, int save_num, double t, hipStream_t compute) {
         hip_ok(hipEventRecord(s.snap, compute), "hipEventRecord");
         hip_ok(hipStreamWaitEvent(copy_, s.snap, 0), "hipStreamWaitEvent");
         const int fields = velocity_ ? 3 : 1;
@@ -191,7 +202,7 @@ class AsyncWriter {
         hip_ok(hipEventRecord(s.copied, copy_), "hipEventRecord");
         {
             std::lock_guard<std::mutex> lk(mu_);
-            jobs_.push_back({&s, save_num});
+            jobs_.push_back({&s, save_num, t});
         }
         cv_.notify_all();
     }
@@ -200,6 +211,11 @@ class AsyncWriter {
     // fails if a write failed.
     void finish() {
         stop();
+        if (nc_) {
+            const int rc = af_nc_close(nc_);
+            nc_ = nullptr;
+            if (rc != AF_OK && error_.empty()) error_ = std::string("nc_close: ") + af_last_error();
+        }
         if (!error_.empty()) throw Fail{error_};
     }
 
@@ -207,6 +223,7 @@ class AsyncWriter {
     struct Job {
         Slot* slot;
         int save_num;
+        double t;
     };
 
     void stop() {
@@ -244,6 +261,22 @@ class AsyncWriter {
             note(std::string("snapshot copy: ") + hipGetErrorString(e));
             return;
         }
+        if (nc_) {  // async_output.cpp:110-139
+            if (!out_.quiet) {
+                printf("[AsyncIO] Writing t=%.0fs to NetCDF (time_index=%d)...\n", job.t,
+                       nc_records_);
+                fflush(stdout);
+            }
+            const double* const* h = job.slot->host;
+            if (af_nc_append(nc_, job.t, h[0], velocity_ ? h[1] : nullptr,
+                             velocity_ ? h[2] : nullptr) != AF_OK)
+                note(std::string("NetCDF: ") + af_last_error());
+            else if (!out_.quiet) {
+                printf("[AsyncIO] Written t=%.0fs to NetCDF, synced.\n", job.t);
+                fflush(stdout);
+            }
+            nc_records_++;
+        }
         if (!asc_) return;  // output_format 1: NetCDF only (async_output.cpp:144)
         static const char* names[3] = {"H", "Vx", "Vy"};
         const int fields = velocity_ ? 3 : 1;
@@ -265,6 +298,8 @@ class AsyncWriter {
     const Output& out_;
     const bool velocity_, asc_;
     const size_t n_;
+    af_nc* nc_ = nullptr;
+    int nc_records_ = 0;
     hipStream_t copy_ = nullptr;
     Slot slots_[2];
     std::deque<Job> jobs_;
@@ -399,7 +434,7 @@ Result run_compat(const af_scenario_desc& d, const af_par& par, const Output& ou
                    "snapshot H");
             if (aw->velocity())
                 ComputeVelocity_GPU(sl.dev[1], sl.dev[2], H, DEM, Qx, Qy, nx, ny, dx, d.nodata);
-            aw->submit(sl, save_num, nullptr);
+            aw->submit(sl, save_num, t, nullptr);
             queued_line(out, t, save_num);
         } else {
             char name[64];
@@ -511,7 +546,7 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
             af_ok(af_batch_snapshot(D, 0, sl.dev[0], v ? sl.dev[1] : nullptr,
                                     v ? sl.dev[2] : nullptr, nullptr),
                   "snapshot");
-            aw->submit(sl, save_num, nullptr);
+            aw->submit(sl, save_num, info.t, nullptr);
             queued_line(out, info.t, save_num);
         } else {
             char name[64];
@@ -619,14 +654,15 @@ int main(int argc, char** argv) {
         Stages stages;
         stages.open(c, par, d.nx);
         out.stages = &stages;
-        if (par.output_format >= 1)
-            printf("Note: output_format %d asks for NetCDF, which this build does not write%s\n",
-                   par.output_format,
-                   par.async_output && par.output_format == 1 ? " (no periodic saves)" : "");
+        if (par.output_format >= 1 && !par.async_output)
+            printf("Note: output_format %d: NetCDF is written by the asynchronous writer only "
+                   "(async_output 1), as in the reference; none is written\n",
+                   par.output_format);
         const auto w0 = std::chrono::steady_clock::now();
         std::unique_ptr<AsyncWriter> aw;
         if (par.async_output)
-            aw.reset(new AsyncWriter(out, par.voutput != 0, par.output_format != 1));
+            aw.reset(new AsyncWriter(out, par.voutput != 0, par.output_format != 1,
+                                     par.output_format >= 1));
         Result r = path == "compat"
                        ? run_compat(d, par, out, aw.get())
                        : run_fused(d, par, out, path == "fp64" ? AF_FP64 : AF_FP32, aw.get());
@@ -650,6 +686,8 @@ int main(int argc, char** argv) {
         printf("Wall clock time: %.2f s (%.2f min)\n", wall, wall / 60.0);
         printf("Final volume: %.0f m3\n", vol);
         printf("Output saved to %s/\n", par.output_dir);
+        if (par.output_format >= 1 && par.async_output)
+            printf("NetCDF output: %s/output.nc\n", par.output_dir);
     } catch (const Fail& f) {
         fprintf(stderr, "lisflood_amd: %s\n", f.msg.c_str());
         af_case_free(c);

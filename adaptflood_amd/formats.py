@@ -170,6 +170,31 @@ def write_asc_async(path, data, xll=0.0, yll=0.0, cellsize=1.0, nodata=-9999.0):
          float(cellsize), float(nodata))
 
 
+class NetCDFWriter:
+    """The asynchronous writer's output.nc (src/async_output.cpp:18-54,
+    108-141) in the classic 64-bit-offset NetCDF format: append(t, H[, Vx,
+    Vy]) adds one record of float grids."""
+
+    def __init__(self, path, ncols, nrows, xll=0.0, yll=0.0, cellsize=1.0, nodata=-9999.0,
+                 velocity=False):
+        h = C.c_void_p()
+        call("af_nc_create", _b(path), int(ncols), int(nrows), float(xll), float(yll),
+             float(cellsize), float(nodata), 1 if velocity else 0, C.byref(h))
+        self.handle, self.shape, self.velocity = h, (nrows, ncols), velocity
+
+    def append(self, t, H, Vx=None, Vy=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (H, Vx, Vy) if a is not None]
+        if any(a.size != self.shape[0] * self.shape[1] for a in arrs):
+            raise ValueError("grid size does not match the file")
+        ptrs = [_lib.dptr(a) for a in arrs] + [None] * (3 - len(arrs))
+        call("af_nc_append", self.handle, float(t), *ptrs)
+
+    def close(self):
+        if self.handle:
+            call("af_nc_close", self.handle)
+            self.handle = None
+
+
 def write_case(dirpath, scn, name="case", xll=0.0, yll=0.0, rain_header="rain", extra_par=""):
     """Writes a Scenario in the reference's input formats (the inverse of
     load_case): .par, .dem.ascii, .bci, .bdy and an optional rain file.

@@ -346,6 +346,21 @@ double af_interp_series(const double* times, const double* values, int32_t n, do
 /* ESRI-ASCII writer in the format of write_asc (src/io.cpp:314-334). */
 int af_write_asc(const char* path, const double* data, int32_t ncols, int32_t nrows, double xll,
                  double yll, double cellsize);
+/* NetCDF output of the asynchronous writer (src/async_output.cpp:18-54
+ * creates the file, :108-141 appends one save): dimensions time (unlimited),
+ * x, y; variables time (double, seconds), H(time, y, x) (float, units "m",
+ * long_name "water_depth", _FillValue = (float)nodata) and, with velocity,
+ * Vx and Vy (float, "m/s"); global xllcorner, yllcorner, cellsize.  The file
+ * is in the classic 64-bit-offset NetCDF format (the reference writes
+ * NetCDF-4 through libnetcdf); append converts the double grids to float as
+ * the reference does and updates the record count, so the file is whole
+ * after every call. */
+typedef struct af_nc af_nc;
+int af_nc_create(const char* path, int32_t ncols, int32_t nrows, double xll, double yll,
+                 double cellsize, double nodata, int32_t velocity, af_nc** out);
+int af_nc_append(af_nc* nc, double t, const double* H, const double* Vx, const double* Vy);
+int af_nc_close(af_nc* nc);
+
 /* ESRI-ASCII writer in the format of the asynchronous writer
  * (AsyncOutputManager::write_asc_file, src/async_output.cpp:158-176):
  * corners "%.6f", NODATA_value the grid's own value as "%.0f". */

@@ -324,3 +324,48 @@ def test_write_asc_async_format(tmp_path):
                       "yllcorner     -7.125000\n", "cellsize      2.0\n", "NODATA_value  -32768\n"]
     assert ls[2:4] == ["xllcorner     1234.5\n", "yllcorner     -7.1\n"]
     assert la[6:] == ls[6:] == ["0.0000 1.2346 -0.0000 \n", "-9999.0000 2.5000 0.0000 \n"]
+
+
+def test_netcdf_output_reads_back(tmp_path):
+    """output.nc of the asynchronous writer (async_output.cpp:18-54, 108-141),
+    read back with an independent classic-NetCDF reader (scipy): dimensions,
+    variables, attributes, and float records equal to the float32 of the
+    grids appended, the record count following each append."""
+    from scipy.io import netcdf_file
+    rng = np.random.default_rng(3)
+    ny, nx = 7, 11
+    path = str(tmp_path / "output.nc")
+    w = formats.NetCDFWriter(path, nx, ny, xll=1234.5, yll=-7.25, cellsize=2.0, nodata=-9999.0,
+                             velocity=True)
+    recs = []
+    for t in (0.0, 600.5, 1201.25):
+        H, Vx, Vy = (rng.normal(0, 2, (ny, nx)) for _ in range(3))
+        w.append(t, H, Vx, Vy)
+        recs.append((t, H, Vx, Vy))
+        with netcdf_file(path, "r", mmap=False) as f:   # whole after every append
+            assert f.variables["time"].shape == (len(recs),)
+    w.close()
+    with netcdf_file(path, "r", mmap=False) as f:
+        assert f.version_byte == 2
+        assert f.dimensions == {"time": None, "x": nx, "y": ny}
+        assert (f.xllcorner, f.yllcorner, f.cellsize) == (1234.5, -7.25, 2.0)
+        tv, Hv = f.variables["time"], f.variables["H"]
+        assert tv.units == b"seconds" and tv.dimensions == ("time",)
+        assert Hv.dimensions == ("time", "y", "x") and Hv.typecode() == "f"
+        assert Hv.units == b"m" and Hv.long_name == b"water_depth"
+        assert Hv._FillValue == np.float32(-9999.0)
+        assert f.variables["Vx"].long_name == b"x_direction_velo"
+        assert f.variables["Vy"].units == b"m/s"
+        assert list(tv[:]) == [r[0] for r in recs]
+        for k, (t, H, Vx, Vy) in enumerate(recs):
+            assert np.array_equal(Hv[k], H.astype(np.float32))
+            assert np.array_equal(f.variables["Vx"][k], Vx.astype(np.float32))
+            assert np.array_equal(f.variables["Vy"][k], Vy.astype(np.float32))
+    # without velocity: time and H only
+    p2 = str(tmp_path / "h_only.nc")
+    w = formats.NetCDFWriter(p2, nx, ny)
+    w.append(5.0, np.ones((ny, nx)))
+    w.close()
+    with netcdf_file(p2, "r", mmap=False) as f:
+        assert set(f.variables) == {"time", "H"}
+        assert np.array_equal(f.variables["H"][0], np.ones((ny, nx), np.float32))

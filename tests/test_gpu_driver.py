@@ -121,8 +121,9 @@ def test_async_and_sync_saves(gpu, tmp_path, path):
     """async_output 1 (the reference default) writes the saves through the
     writer thread in AsyncOutputManager's format (corners %.6f, the grid's
     NODATA value) with Vx/Vy beside each H; async_output 0 writes write_asc's
-    format and H only.  The grids of every save agree to the last digit;
-    output_format 1 (NetCDF) in async mode writes no ASC saves."""
+    format and H only.  The grids of every save agree to the last digit.
+    output_format 2 adds output.nc, whose records hold the same grids;
+    output_format 1 writes output.nc and no ASC saves."""
     runs = {}
     for mode in (1, 0):
         sub = tmp_path / f"m{mode}"
@@ -152,11 +153,29 @@ def test_async_and_sync_saves(gpu, tmp_path, path):
     for f in ("H_final.asc", "Vx_final.asc", "Vy_final.asc"):
         with open(os.path.join(oa, f)) as a, open(os.path.join(os_, f)) as b:
             assert a.read() == b.read(), f
-    sub = tmp_path / "nc"
-    sub.mkdir()
-    par = _write_small_case(sub, "saveint 60\noutput_format 1\n")
-    r = subprocess.run([EXE, "--path", path, "--quiet", par], capture_output=True, text=True,
-                       timeout=120)
-    assert r.returncode == 0, r.stderr
-    out = str(sub / "case" / "results")
-    assert _saves(out, "H") == [] and os.path.exists(os.path.join(out, "H_final.asc"))
+    # output_format 2: output.nc (one float record per save, t = 0 included)
+    # beside the ASC saves; output_format 1: output.nc only
+    from scipy.io import netcdf_file
+    for fmt in (2, 1):
+        sub = tmp_path / f"nc{fmt}"
+        sub.mkdir()
+        par = _write_small_case(sub, f"saveint 60\nvoutput\noutput_format {fmt}\n")
+        r = subprocess.run([EXE, "--path", path, par], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = str(sub / "case" / "results")
+        assert "NetCDF file created:" in r.stdout and "NetCDF output:" in r.stdout
+        with netcdf_file(os.path.join(out, "output.nc"), "r", mmap=False) as f:
+            times = np.array(f.variables["time"][:])
+            Hn = np.array(f.variables["H"][:])
+            Vxn = np.array(f.variables["Vx"][:])
+        assert len(times) == len(ha) and times[0] == 0.0 and np.all(np.diff(times) > 0)
+        if fmt == 1:
+            assert _saves(out, "H") == [] and os.path.exists(os.path.join(out, "H_final.asc"))
+            continue
+        assert _saves(out, "H") == ha
+        for k, f in enumerate(ha):          # the ASC grids carry 4 decimals
+            _, da = _read_asc(os.path.join(out, f))
+            _, va = _read_asc(os.path.join(out, f.replace("H_", "Vx_")))
+            assert np.max(np.abs(Hn[k] - da)) <= 5.1e-5, f
+            assert np.max(np.abs(Vxn[k] - va)) <= 5.1e-5 + 1e-6 * np.max(np.abs(va)), f
