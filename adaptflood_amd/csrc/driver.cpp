@@ -559,12 +559,12 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     save();
 
     auto show_reports = [&]() {
-        af_run_info info;
-        af_ok(af_domain_info(D, &info, nullptr), "info");
-        if (info.n_reports <= shown) return;
-        rep.assign(4 * (size_t)info.n_reports, 0.0);
+        af_run_info now;
+        af_ok(af_domain_info(D, &now, nullptr), "info");
+        if (now.n_reports <= shown) return;
+        rep.assign(4 * (size_t)now.n_reports, 0.0);
         int32_t got = 0;
-        af_ok(af_domain_read_reports(D, rep.data(), info.n_reports, &got, nullptr), "reports");
+        af_ok(af_domain_read_reports(D, rep.data(), now.n_reports, &got, nullptr), "reports");
         for (int i = shown; i < got; i++)
             report_line(rep[4 * i], rep[4 * i + 1], rep[4 * i + 2], rep[4 * i + 3]);
         shown = got;
