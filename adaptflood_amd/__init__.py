@@ -5,7 +5,9 @@ Layers:
   kernels     the reference's per-function call surface (FP64, reference layout)
   domain      fused single-kernel step for one scenario or a ragged batch
   serve       continuous-batching request loop over a device-resident pool
-  formats     .par / .dem.ascii / .bci / .bdy / rain readers and the ASC writer
+  tune        autotuner for the tile shape, kernel variant and tile order
+  formats     .par / .dem.ascii / .bci / .bdy / rain / stage readers, the two
+              ESRI-ASCII writers and the NetCDF writer
   scenario    host-side scenario description
   generators  seeded synthetic workloads
 """
