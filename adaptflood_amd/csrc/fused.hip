@@ -89,7 +89,7 @@ namespace af {
 namespace {
 
 constexpr int kVolBlocks = 1024;
-constexpr int kMinTiles = 1024;   // wavefronts a lone scenario should offer
+constexpr int kMinTiles = 2048;   // wavefronts a lone scenario should offer
 
 int elem_size(int precision) { return precision == AF_FP64 ? 8 : 4; }
 

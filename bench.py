@@ -465,6 +465,84 @@ def bench_divergence(args):
     print(json.dumps(out), flush=True)
 
 
+def _asc_body(path):
+    """The grid lines of an ESRI-ASCII file (the six header lines dropped)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    cut = 0
+    for _ in range(6):
+        cut = data.index(b"\n", cut) + 1
+    return data[cut:]
+
+
+def bench_driver(args):
+    """lisflood_amd, the reference program rebuilt on this library, end to
+    end on the 4096^2 valley case written in the reference's input formats:
+    parse, upload, step to --drv-time, save every --save-int s with
+    velocities, write the final grids.  It runs once with the asynchronous
+    save path (the reference's default) and once with the synchronous one,
+    and checks that both leave the same grids."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    from adaptflood_amd import formats, generators
+    exe = os.path.join(ROOT, "adaptflood_amd", "_lib", "lisflood_amd")
+    if not os.path.exists(exe):
+        print("lisflood_amd is not built (run __graft_entry__.build())", file=sys.stderr)
+        sys.exit(1)
+    scn = generators.valley_case(n=args.valley_size)
+    scn.sim_time = args.drv_time
+    tmp = tempfile.mkdtemp(prefix="af_driver_")
+    try:
+        t = time.perf_counter()
+        base = formats.write_case(os.path.join(tmp, "case"), scn)
+        print(f"[driver] {scn.nx}x{scn.ny} case written in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
+        with open(base) as f:
+            text = f.read()
+        runs = {}
+        for mode in (1, 0):
+            out = os.path.join(tmp, f"out{mode}")
+            par = os.path.join(tmp, f"run{mode}.par")
+            with open(par, "w") as f:   # later keys override the case's own
+                f.write(text + f"dirroot {out}\nsaveint {args.save_int!r}\nvoutput\n"
+                        f"async_output {mode}\n")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "--path", args.precision, "--quiet", par],
+                               capture_output=True, text=True, timeout=900)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                print(r.stdout[-2000:], r.stderr[-2000:], file=sys.stderr)
+                sys.exit(1)
+            loop = float(re.search(r"Wall clock time: ([\d.]+) s", r.stdout).group(1))
+            steps = int(re.search(r"Steps: (\d+)", r.stdout).group(1))
+            runs[mode] = {"process_s": round(wall, 2), "loop_s": loop, "steps": steps,
+                          "loop_gcups": round(scn.cells * steps / loop / 1e9, 2),
+                          "files": sorted(os.listdir(out)), "dir": out}
+            print(f"[driver] async_output {mode}: process {wall:.1f}s, loop {loop:.2f}s, "
+                  f"{steps} steps, {len(runs[mode]['files'])} files", file=sys.stderr, flush=True)
+        a, b = runs[1], runs[0]
+        grids = [f for f in b["files"] if f.endswith(".asc")]
+        same = all(_asc_body(os.path.join(a["dir"], f)) == _asc_body(os.path.join(b["dir"], f))
+                   for f in grids)
+        extra = sorted(set(a["files"]) - set(b["files"]))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    for r in runs.values():
+        del r["files"], r["dir"]
+    print(json.dumps({
+        "metric": "lisflood_amd end to end (parse, step, saves, final grids)",
+        "value": runs[1]["process_s"], "unit": "s", "higher_is_better": False,
+        "config": {"workload": f"valley {scn.nx}x{scn.ny} at {scn.dx:g} m, {len(scn.points)} "
+                               f"points, rain; sim_time {scn.sim_time:g} s, saves every "
+                               f"{args.save_int:g} s with velocity",
+                   "precision": args.precision},
+        "async": runs[1], "sync": runs[0],
+        "grids_equal_across_modes": same, "grids_compared": len(grids),
+        "async_only_files": len(extra)}), flush=True)
+
+
 def bench_shard(args):
     """One GPU's share of the 16384^2 roofline case split into `--parts` row
     slabs: slab 0 (16384 x 2048 at 8 parts) with its own point sources.  The
@@ -521,7 +599,8 @@ def main():
                     help="also run a 4 GiB device-to-device copy (the known-size kernel that "
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
-                    choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard"),
+                    choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard",
+                             "driver"),
                     default="roofline")
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
@@ -529,6 +608,9 @@ def main():
                     help="accuracy: simulated seconds of the valley comparison with the oracle")
     ap.add_argument("--roof-time", type=float, default=3600.0,
                     help="accuracy: simulated seconds of the roofline-case FP32/FP64 comparison")
+    ap.add_argument("--drv-time", type=float, default=7200.0,
+                    help="driver: simulated seconds of the valley case")
+    ap.add_argument("--save-int", type=float, default=600.0, help="driver: seconds between saves")
     ap.add_argument("--div-cases", default="valley,roofline",
                     help="divergence: comma-separated cases to run (valley, roofline)")
     ap.add_argument("--acc-cases", default="valley,roofline",
@@ -563,6 +645,8 @@ def main():
         return bench_divergence(args)
     if args.workload == "shard":
         return bench_shard(args)
+    if args.workload == "driver":
+        return bench_driver(args)
     served_reqs = None
     if not args.no_served:
         # generated in worker processes before this process touches the GPU
