@@ -179,7 +179,10 @@ typedef struct af_run_info {
 typedef struct af_domain af_domain; /* opaque, host side */
 
 /* Tile shape: each wavefront owns a strip of 64*V columns (V = 4 at FP32,
- * 2 at FP64) and marches down `tile_rows` rows.  0 = library default. */
+ * 2 at FP64) and marches down `tile_rows` rows.  0 = library default: 64 rows
+ * at FP32 and 32 at FP64, halved per scenario while the scenario would offer
+ * fewer than 2048 wavefronts, down to 8 rows.  No field changes any result:
+ * the tile shape, variant and order only change how the work is split. */
 typedef struct af_tuning {
     int32_t tile_rows;
     int32_t waves_per_block; /* 1 or 4 */
