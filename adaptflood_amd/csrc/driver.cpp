@@ -94,6 +94,14 @@ struct Stages {
         }
         fprintf(fp, "\n");
         interval = par.stage_int;
+        printf("Stage output file: %s\n", path.c_str());
+    }
+    // Closes the CSV; true when one was open.
+    bool close() {
+        if (!fp) return false;
+        fclose(fp);
+        fp = nullptr;
+        return true;
     }
     // writes one line from the gauge values
     void line(double t, const std::vector<double>& v) {
@@ -635,25 +643,54 @@ int main(int argc, char** argv) {
     if (!parfile || (path != "compat" && path != "fp64" && path != "fp32")) return usage(argv[0]);
     af_case* c = af_case_new();
     try {
-        printf("=== LISFLOOD-GPU Flood Simulation (MI355X, %s path) ===\n", path.c_str());
+        // console lines of main.cpp:19-46, 70-86
+        printf("=== LISFLOOD-GPU Flood Simulation ===\n");
+        printf("Execution space: HIP gfx950 (%s path)\n\n", path.c_str());
         printf("Reading parameter file: %s\n", parfile);
-        af_ok(af_case_load(c, parfile), "load");
         af_par par;
+        af_ok(af_read_par(parfile, &par), "load");
+        static const char* fmt_str[] = {"ASC", "NetCDF", "Both"};
+        printf("Parameters:\n");
+        printf("  DEM file:    %s\n", par.dem_file);
+        printf("  BCI file:    %s\n", par.bci_file);
+        printf("  BDY file:    %s\n", par.bdy_file);
+        printf("  Output dir:  %s\n", par.output_dir);
+        printf("  Sim time:    %.0f s\n", par.sim_time);
+        printf("  Save int:    %.0f s\n", par.save_int);
+        printf("  CFL:         %.2f (adaptive timestep)\n", par.cfl);
+        printf("  Manning n:   %.4f\n", par.fpn);
+        printf("  Output fmt:  %s\n",
+               par.output_format >= 0 && par.output_format <= 2 ? fmt_str[par.output_format]
+                                                                 : "?");
+        printf("  Velocity:    %s\n", par.voutput ? "enabled" : "disabled");
+        printf("  Async I/O:   %s\n", par.async_output ? "enabled" : "disabled");
+        printf("\n");
+        fflush(stdout);
+        af_ok(af_case_load(c, parfile), "load");
         af_ok(af_case_par(c, &par), "par");
         af_scenario_desc d;
         af_ok(af_case_scenario(c, &d), "scenario");
+        if (d.west_free || d.east_free || d.north_free || d.south_free)
+            printf("Line FREE boundaries: W=%d E=%d N=%d S=%d\n", d.west_free ? 1 : 0,
+                   d.east_free ? 1 : 0, d.north_free ? 1 : 0, d.south_free ? 1 : 0);
+        size_t n_flow = 0;
+        for (int p = 0; p < d.n_points; p++)
+            if (d.pt_type[p] == AF_BC_QFIX || d.pt_type[p] == AF_BC_QVAR) n_flow++;
+        printf("Flow boundaries: %zu, Other boundaries: %zu\n", n_flow,
+               (size_t)d.n_points - n_flow);
         Output out;
         out.quiet = quiet;
         out.dir = par.output_dir;
         af_ok(af_case_dem_header(c, &out.g.nx, &out.g.ny, &out.g.xll, &out.g.yll, &out.g.cell,
                                  &out.g.nodata),
               "dem header");
-        printf("Grid: %d x %d, cellsize %.3f, sim time %.0f s, save every %.0f s\n", d.nx, d.ny,
-               d.dx, d.sim_time, par.save_int);
         mkdir(par.output_dir, 0755);
         Stages stages;
         stages.open(c, par, d.nx);
         out.stages = &stages;
+        printf("\nStarting simulation: %.0f seconds, output every %.0f seconds\n\n", d.sim_time,
+               par.save_int);
+        fflush(stdout);
         if (par.output_format >= 1 && !par.async_output)
             printf("Note: output_format %d: NetCDF is written by the asynchronous writer only "
                    "(async_output 1), as in the reference; none is written\n",
@@ -679,6 +716,7 @@ int main(int argc, char** argv) {
             std::vector<double> dem(d.dem, d.dem + (size_t)d.nx * d.ny);
             velocity_files(out, r.H, dem, r.Qx, r.Qy, d.dx);
         }
+        if (stages.close()) printf("Stage output saved.\n");  // main.cpp:390-394
         double vol, area;
         mass_of(r.H, d.dx * d.dx, &vol, &area);
         printf("\n=== Simulation Complete ===\n");

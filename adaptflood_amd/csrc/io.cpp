@@ -125,6 +125,9 @@ int af_read_par(const char* path, af_par* p) {
         af::set_error("Cannot open parameter file %s", path);
         return AF_E_IO;
     }
+    // keys the file leaves out keep the defaults of types.hpp:18-38, as the
+    // reference's default-constructed Parameters does
+    af_par_defaults(p);
     std::string line;
     int lineno = 0;
     while (std::getline(f, line)) {

@@ -305,7 +305,9 @@ typedef struct af_case af_case; /* opaque parsed case */
 
 /* Parameters with the defaults of src/types.hpp:18-38. */
 void af_par_defaults(af_par* p);
-/* Replaces read_par (src/io.cpp:9-47); keys as the code reads them. */
+/* Replaces read_par (src/io.cpp:9-47); keys as the code reads them.  `p` is
+ * first set to the defaults (af_par_defaults), then overwritten by the keys the
+ * file holds; on an error its contents are unspecified. */
 int af_read_par(const char* path, af_par* p);
 
 af_case* af_case_new(void);

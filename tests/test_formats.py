@@ -369,3 +369,18 @@ def test_netcdf_output_reads_back(tmp_path):
     with netcdf_file(p2, "r", mmap=False) as f:
         assert set(f.variables) == {"time", "H"}
         assert np.array_equal(f.variables["H"][0], np.ones((ny, nx), np.float32))
+
+
+def test_read_par_starts_from_the_defaults(tmp_path):
+    """A caller's uninitialised parameter struct: keys the file leaves out take
+    the defaults of types.hpp:18-38, as the reference's Parameters does."""
+    import ctypes
+    from adaptflood_amd import _lib
+    p = _lib.Par()
+    ctypes.memset(ctypes.byref(p), 0xA5, ctypes.sizeof(p))   # garbage, no defaults applied
+    path = _w(tmp_path, "a.par", "sim_time 120\nsaveint 30\n")
+    assert _lib.lib().af_read_par(path.encode(), ctypes.byref(p)) == 0
+    assert (p.sim_time, p.save_int) == (120.0, 30.0)
+    assert p.async_output == 1 and p.voutput == 0 and p.output_format == 0
+    assert (p.cfl, p.fpn, p.mass_int) == (0.7, 0.035, 60.0)
+    assert p.output_dir == b"results" and p.dem_file == b"dem.ascii"

@@ -75,6 +75,14 @@ def test_driver_paths(gpu, tmp_path, path, tol):
     r = subprocess.run([EXE, "--path", path, "--quiet", par], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr
+    # the console lines of main.cpp:19-46, 70-86, 109, 163, 393
+    n_flow = sum(1 for p in back.points if p.type in (oracle.QFIX, oracle.QVAR))
+    for line in ("=== LISFLOOD-GPU Flood Simulation ===", "Parameters:", "  Sim time:    300 s",
+                 "  Save int:    100 s", "  Velocity:    enabled", "  Async I/O:   enabled",
+                 f"Flow boundaries: {n_flow}, Other boundaries: {len(back.points) - n_flow}",
+                 "Starting simulation: 300 seconds, output every 100 seconds",
+                 "Stage output saved.", "=== Simulation Complete ==="):
+        assert line in r.stdout.splitlines(), line
     out = str(tmp_path / "case" / "results")
     _, H = _read_asc(os.path.join(out, "H_final.asc"))
     err = float(np.max(np.abs(H - ref["H"])))
