@@ -129,6 +129,7 @@ SIGNATURES = [
     ("af_case_read_bdy", _I, [_V, C.c_char_p]),
     ("af_case_read_rain", _I, [_V, C.c_char_p]),
     ("af_case_load", _I, [_V, C.c_char_p]),
+    ("af_case_load_par", _I, [_V, C.POINTER(Par)]),
     ("af_case_scenario", _I, [_V, C.POINTER(ScenarioDesc)]),
     ("af_case_par", _I, [_V, C.POINTER(Par)]),
     ("af_case_dem_header", _I, [_V, _ip, _ip, _dp, _dp, _dp, _dp]),

@@ -1,8 +1,10 @@
 // driver.cpp -- lisflood_amd: the reference program (src/main.cpp) over the
 // MI355X library.
 //
-//   lisflood_amd [--path compat|fp64|fp32] [--quiet] <parfile>
+//   lisflood_amd [--path compat|fp64|fp32] [--quiet] [parfile]
 //
+// Without a parameter file it runs on the defaults, reading dem.ascii,
+// bci.bci and bdy.bdy from the working directory (main.cpp:28-31).
 // --path compat (default): the reference's own call sequence
 //   (main.cpp:227-288) through the drop-in wrappers of adaptflood_compat.hpp,
 //   FP64 on the reference memory layout, one synchronous call per kernel.
@@ -624,7 +626,7 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
 }
 
 int usage(const char* argv0) {
-    fprintf(stderr, "usage: %s [--path compat|fp64|fp32] [--quiet] <parfile>\n", argv0);
+    fprintf(stderr, "usage: %s [--path compat|fp64|fp32] [--quiet] [parfile]\n", argv0);
     return 2;
 }
 
@@ -640,15 +642,21 @@ int main(int argc, char** argv) {
         else if (argv[i][0] == '-') return usage(argv[0]);
         else parfile = argv[i];
     }
-    if (!parfile || (path != "compat" && path != "fp64" && path != "fp32")) return usage(argv[0]);
+    if (path != "compat" && path != "fp64" && path != "fp32") return usage(argv[0]);
     af_case* c = af_case_new();
     try {
         // console lines of main.cpp:19-46, 70-86
         printf("=== LISFLOOD-GPU Flood Simulation ===\n");
         printf("Execution space: HIP gfx950 (%s path)\n\n", path.c_str());
-        printf("Reading parameter file: %s\n", parfile);
         af_par par;
-        af_ok(af_read_par(parfile, &par), "load");
+        if (parfile) {
+            printf("Reading parameter file: %s\n", parfile);
+            af_ok(af_read_par(parfile, &par), "load");
+        } else {  // main.cpp:28-31: the defaults, inputs from the working directory
+            printf("Usage: %s <parfile>\n", argv[0]);
+            printf("Using default parameters...\n\n");
+            af_par_defaults(&par);
+        }
         static const char* fmt_str[] = {"ASC", "NetCDF", "Both"};
         printf("Parameters:\n");
         printf("  DEM file:    %s\n", par.dem_file);
@@ -666,7 +674,7 @@ int main(int argc, char** argv) {
         printf("  Async I/O:   %s\n", par.async_output ? "enabled" : "disabled");
         printf("\n");
         fflush(stdout);
-        af_ok(af_case_load(c, parfile), "load");
+        af_ok(af_case_load_par(c, &par), "load");
         af_ok(af_case_par(c, &par), "par");
         af_scenario_desc d;
         af_ok(af_case_scenario(c, &d), "scenario");

@@ -571,8 +571,20 @@ int af_case_load(af_case* c, const char* par_path) {
         af::set_error("case_load: null argument");
         return AF_E_INVALID;
     }
-    int rc = af_read_par(par_path, &c->par);
+    af_par par;
+    const int rc = af_read_par(par_path, &par);
     if (rc) return rc;
+    return af_case_load_par(c, &par);
+}
+
+// The inputs a parameter set names, in the order of main.cpp:48-113.
+int af_case_load_par(af_case* c, const af_par* par) {
+    if (!c || !par) {
+        af::set_error("case_load: null argument");
+        return AF_E_INVALID;
+    }
+    c->par = *par;
+    int rc;
     if ((rc = af_case_read_dem(c, c->par.dem_file))) return rc;
     if ((rc = af_case_read_bci(c, c->par.bci_file))) return rc;
     if ((rc = af_case_read_bdy(c, c->par.bdy_file))) return rc;

@@ -329,6 +329,11 @@ int af_case_stage(af_case* c, int32_t i, int32_t* grid_x, int32_t* grid_y, char*
 /* Runs the input sequence of main.cpp:24-113 for a .par file: par, DEM, BCI,
  * BDY (all required), the optional rain file and the optional stage file. */
 int af_case_load(af_case* c, const char* par_path);
+/* Loads the inputs a filled parameter set names (DEM, BCI, BDY, and the rain
+ * and stage files when set), as af_case_load does after reading its .par
+ * file.  main.cpp:26-31 runs this way on the defaults (af_par_defaults) when
+ * no .par file is given. */
+int af_case_load_par(af_case* c, const af_par* par);
 /* Fills `desc` with pointers into the case (valid while the case lives),
  * with the constants of main.cpp:115-127. */
 int af_case_scenario(af_case* c, af_scenario_desc* desc);

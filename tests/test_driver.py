@@ -16,11 +16,17 @@ def exe():
     return EXE
 
 
-def test_usage(exe):
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "usage" in r.stderr
+def test_usage(exe, tmp_path):
     r = subprocess.run([exe, "--path", "fp16", "x.par"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
+    # no .par file: the defaults, inputs from the working directory (main.cpp:28-31);
+    # an empty directory has no dem.ascii, which ends the run before any GPU work
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert "Using default parameters..." in r.stdout
+    assert "  DEM file:    dem.ascii" in r.stdout and "  Sim time:    3600 s" in r.stdout
+    assert r.returncode == 1 and "lisflood_amd: load" in r.stderr
 
 
 def test_missing_par_file(exe, tmp_path):

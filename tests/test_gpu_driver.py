@@ -187,3 +187,26 @@ def test_async_and_sync_saves(gpu, tmp_path, path):
             _, va = _read_asc(os.path.join(out, f.replace("H_", "Vx_")))
             assert np.max(np.abs(Hn[k] - da)) <= 5.1e-5, f
             assert np.max(np.abs(Vxn[k] - va)) <= 5.1e-5 + 1e-6 * np.max(np.abs(va)), f
+
+
+def test_runs_on_defaults_without_a_par_file(gpu, tmp_path):
+    """No .par file: the defaults of types.hpp, dem.ascii / bci.bci / bdy.bdy
+    from the working directory and results/ as output (main.cpp:28-31); the
+    run matches the oracle on the same inputs and defaults."""
+    scn = generators.small_case(30, 22, seed=29, sim_time=3600.0)
+    scn.fpn, scn.cfl, scn.depth_thresh, scn.mass_int = 0.035, 0.7, 0.001, 60.0
+    par = formats.write_case(str(tmp_path / "w"), scn)
+    back = formats.load_case(par).scenario()
+    for src, dst in (("case.dem.ascii", "dem.ascii"), ("case.bci", "bci.bci"),
+                     ("case.bdy", "bdy.bdy")):
+        os.replace(str(tmp_path / "w" / src), str(tmp_path / "w" / dst))
+    r = subprocess.run([EXE, "--path", "fp64", "--quiet"], capture_output=True, text=True,
+                       timeout=120, cwd=str(tmp_path / "w"))
+    assert r.returncode == 0, r.stderr
+    assert "Using default parameters..." in r.stdout
+    back.rain_times = back.rain_rates = None          # the defaults name no rain file
+    back.sim_time = 3600.0
+    ref = oracle.run(back)
+    _, H = _read_asc(str(tmp_path / "w" / "results" / "H_final.asc"))
+    assert int(re.search(r"Steps: (\d+)", r.stdout).group(1)) == ref["steps"]
+    assert float(np.max(np.abs(H - ref["H"]))) <= 1e-4
