@@ -537,6 +537,9 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     af_tuning tun{};
     const double nrep = d.mass_int > 0 ? std::ceil(d.sim_time / d.mass_int) + 4 : 4;
     tun.report_cap = (int32_t)std::min(nrep, 1048576.0);
+    // dt of the first 4M steps, for the Step lines
+    const bool step_lines = !out.quiet;
+    tun.dt_log_cap = step_lines ? (1 << 22) : 0;
     size_t bytes = 0;
     af_ok(af_domain_arena_bytes(&d, precision, &tun, &bytes), "arena_bytes");
     void* arena = nullptr;
@@ -568,16 +571,44 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     };
     save();
 
-    auto show_reports = [&]() {
+    // The "Step" lines of main.cpp:238-241 come from the device dt log, in
+    // step order with the mass-report lines: the host sums the logged dt in
+    // sequence, as the device advances t, so the times agree exactly and a
+    // report (recorded at the t that ends its step) follows its step's line.
+    std::vector<double> dts;
+    long long printed = 0;  // steps whose lines are out
+    double t_run = 0.0;     // start time of step `printed`
+    auto show_progress = [&]() {
         af_run_info now;
         af_ok(af_domain_info(D, &now, nullptr), "info");
-        if (now.n_reports <= shown) return;
-        rep.assign(4 * (size_t)now.n_reports, 0.0);
-        int32_t got = 0;
-        af_ok(af_domain_read_reports(D, rep.data(), now.n_reports, &got, nullptr), "reports");
-        for (int i = shown; i < got; i++)
-            report_line(rep[4 * i], rep[4 * i + 1], rep[4 * i + 2], rep[4 * i + 3]);
-        shown = got;
+        int32_t nrep = shown;
+        if (now.n_reports > shown) {
+            rep.assign(4 * (size_t)now.n_reports, 0.0);
+            af_ok(af_domain_read_reports(D, rep.data(), now.n_reports, &nrep, nullptr),
+                  "reports");
+        }
+        int64_t logged = 0;
+        if (step_lines) {
+            af_ok(af_domain_read_dt_log(D, nullptr, 0, &logged, nullptr), "dt log");
+            dts.resize((size_t)logged);
+            if (logged > printed)
+                af_ok(af_domain_read_dt_log(D, dts.data(), logged, &logged, nullptr), "dt log");
+        }
+        for (; printed < logged; printed++) {
+            const double dt = dts[(size_t)printed];
+            if (printed < 10 || printed % 100 == 0)
+                printf("Step %lld, t=%.2fs, dt=%.4fs\n", printed, t_run, dt);
+            t_run += dt;
+            while (shown < nrep && rep[4 * (size_t)shown] == t_run) {
+                report_line(rep[4 * shown], rep[4 * shown + 1], rep[4 * shown + 2],
+                            rep[4 * shown + 3]);
+                shown++;
+            }
+        }
+        for (; shown < nrep; shown++)  // past the logged steps
+            report_line(rep[4 * shown], rep[4 * shown + 1], rep[4 * shown + 2],
+                        rep[4 * shown + 3]);
+        fflush(stdout);
     };
 
     double next_save = par.save_int;
@@ -597,7 +628,7 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
             af_ok(af_domain_run(D, 216, 1, nullptr), "run");
         }
         af_ok(af_domain_info(D, &info, nullptr), "info");
-        show_reports();
+        show_progress();
         if (gauges && info.t >= next_stage) {
             std::vector<double> v(out.stages->idx.size());
             af_ok(af_batch_read_cells(D, 0, out.stages->idx.data(), (int)v.size(), v.data(),

@@ -210,3 +210,22 @@ def test_runs_on_defaults_without_a_par_file(gpu, tmp_path):
     _, H = _read_asc(str(tmp_path / "w" / "results" / "H_final.asc"))
     assert int(re.search(r"Steps: (\d+)", r.stdout).group(1)) == ref["steps"]
     assert float(np.max(np.abs(H - ref["H"]))) <= 1e-4
+
+
+def test_fused_console_matches_reference_sequence(gpu, tmp_path):
+    """Without --quiet the fused FP64 path prints the reference's "Step"
+    lines (from the device dt log) and mass reports in the same order and
+    with the same text as the reference call sequence (compat path)."""
+    scn = generators.small_case(40, 31, seed=37, sim_time=700.0)
+    scn.mass_int = 45.0
+    par = formats.write_case(str(tmp_path / "c"), scn, extra_par="saveint 200\n")
+    lines = {}
+    for path in ("compat", "fp64"):
+        r = subprocess.run([EXE, "--path", path, par], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines[path] = [ln for ln in r.stdout.splitlines()
+                       if ln.startswith("Step ") or REPORT.match(ln)]
+    steps = [ln for ln in lines["compat"] if ln.startswith("Step ")]
+    assert len(steps) > 12 and steps[10].startswith("Step 100,")
+    assert lines["fp64"] == lines["compat"]
