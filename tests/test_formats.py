@@ -384,3 +384,21 @@ def test_read_par_starts_from_the_defaults(tmp_path):
     assert p.async_output == 1 and p.voutput == 0 and p.output_format == 0
     assert (p.cfl, p.fpn, p.mass_int) == (0.7, 0.035, 60.0)
     assert p.output_dir == b"results" and p.dem_file == b"dem.ascii"
+
+
+def test_netcdf_writer_rejects_what_the_format_cannot_hold(tmp_path):
+    """One record's float grid must stay under 2 GiB (readers take the
+    32-bit slab size as signed); bad sizes, paths and missing velocity grids
+    are errors, not files."""
+    from adaptflood_amd._lib import AFError
+    with pytest.raises(AFError):
+        formats.NetCDFWriter(str(tmp_path / "big.nc"), 32768, 16384)     # exactly 2 GiB
+    assert not os.path.exists(str(tmp_path / "big.nc"))
+    with pytest.raises(AFError):
+        formats.NetCDFWriter(str(tmp_path / "no_such_dir" / "o.nc"), 4, 4)
+    w = formats.NetCDFWriter(str(tmp_path / "v.nc"), 4, 3, velocity=True)
+    with pytest.raises(AFError):
+        w.append(0.0, np.zeros((3, 4)))                                  # Vx, Vy missing
+    with pytest.raises(ValueError):
+        w.append(0.0, np.zeros((4, 4)), np.zeros((4, 4)), np.zeros((4, 4)))
+    w.close()
