@@ -118,7 +118,8 @@ class Pool:
 
 
 class Request:
-    __slots__ = ("rid", "scenario", "arrival", "admitted", "done", "steps", "cells", "result")
+    __slots__ = ("rid", "scenario", "arrival", "admitted", "done", "steps", "cells", "result",
+                 "error")
 
     def __init__(self, rid, scenario, arrival):
         self.rid, self.scenario, self.arrival = rid, scenario, arrival
@@ -126,6 +127,7 @@ class Request:
         self.steps = 0
         self.cells = scenario.cells
         self.result = None
+        self.error = None
 
 
 def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, graph=True):
@@ -137,12 +139,17 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
 
     clock="wall": arrivals are seconds on the wall clock from the start of the
     call (a request is admitted once it has arrived); clock="steps": arrivals
-    are pool step indices (a deterministic schedule for tests).  Returns the
-    requests with admission/completion times filled in and a summary."""
+    are pool step indices (a deterministic schedule for tests).  Requests are
+    admitted in arrival order.  One that an empty pool cannot take (larger
+    than its HBM budget, slots or tile table) is rejected with `error` set
+    rather than waited for.  Returns the completed requests, with admission
+    and completion times filled in, and a summary (which counts the rejected
+    ones)."""
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
     waiting = collections.deque()
     active = {}
     done = []
+    rejected = []
     t0 = time.perf_counter()
     step = 0
     cell_updates = 0
@@ -158,7 +165,12 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
             r = waiting[0]
             slot = pool.admit(r.scenario)
             if slot is None:
-                break
+                if active:
+                    break                   # room frees up as active requests finish
+                waiting.popleft()           # an empty pool cannot take it: it never will
+                r.error = "larger than the pool (HBM budget, scenario slots or tile table)"
+                rejected.append(r)
+                continue
             waiting.popleft()
             r.admitted = now()
             active[slot] = r
@@ -187,6 +199,7 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
     lat = np.array([r.done - r.arrival for r in done]) if done else np.zeros(1)
     summary = {
         "requests": len(done),
+        "rejected": len(rejected),
         "wall_s": wall,
         "cell_updates": cell_updates,
         "gcups": cell_updates / wall / 1e9 if clock == "wall" else None,

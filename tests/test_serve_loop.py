@@ -79,3 +79,34 @@ def test_idle_pool_jumps_to_the_next_arrival():
     assert [r.rid for r in done] == [0, 1]
     assert done[1].admitted == 1000
     assert pool.step < 1100   # no stepping through the idle gap
+
+
+class CellPool(FakePool):
+    """FakePool that also holds at most `cells` cells at once."""
+
+    def __init__(self, slots, cells):
+        super().__init__(slots)
+        self.cells = cells
+
+    def admit(self, scn):
+        used = sum(rec[0].cells for rec in self.active.values())
+        if used + scn.cells > self.cells:
+            return None
+        return super().admit(scn)
+
+
+def test_request_larger_than_the_pool_is_rejected_not_waited_for():
+    """A request that even an empty pool cannot hold is rejected with an
+    error; those behind it still run, and the loop ends."""
+    reqs = [Request(0, Scn(500, 10), arrival=0), Request(1, Scn(5000, 10), arrival=1),
+            Request(2, Scn(400, 10), arrival=2), Request(3, Scn(600, 5), arrival=30)]
+    pool = CellPool(slots=4, cells=1000)
+    done, summary = serve(reqs, pool, chunk_steps=5, clock="steps")
+    assert summary["requests"] == 3 and summary["rejected"] == 1
+    assert sorted(r.rid for r in done) == [0, 2, 3]
+    assert reqs[1].error and reqs[1].admitted is None and reqs[1].done is None
+    assert all(r.error is None for r in done) and all(r.steps > 0 for r in done)
+    # admission is in arrival order: request 1 could not be judged too large
+    # while request 0 still held part of the pool, so request 2 waited behind
+    # it until request 0 finished and request 1 was rejected
+    assert reqs[2].admitted >= reqs[0].done
