@@ -94,3 +94,21 @@ def test_served_fp32_strip_widths_equals_alone(gpu):
         assert r.steps == d.info()["steps"], r.rid
         assert np.array_equal(r.result, d.read("H")), r.rid
         assert err <= 1e-3
+
+
+def test_request_over_budget_is_rejected(gpu):
+    """With the real pool: a scenario whose arena extent exceeds the whole HBM
+    budget is rejected; the others are served and equal their solo runs."""
+    small = [generators.small_case(40, 30, seed=120 + k, sim_time=30.0) for k in range(2)]
+    big = generators.small_case(700, 600, seed=130, sim_time=30.0)
+    pool = Pool(budget_bytes=2 << 20, precision="fp32", max_scenarios=4, max_tiles=4096)
+    assert pool.footprint(big)[0] > (2 << 20) > pool.footprint(small[0])[0]
+    reqs = [Request(0, small[0], arrival=0), Request(1, big, arrival=1),
+            Request(2, small[1], arrival=2)]
+    done, summary = serve(reqs, pool, chunk_steps=6, clock="steps", keep_results=True)
+    assert summary["rejected"] == 1 and reqs[1].error
+    assert sorted(r.rid for r in done) == [0, 2]
+    for r in done:
+        d = Domain(r.scenario, precision="fp32")
+        d.run(check_every=12, graph=True)
+        assert np.array_equal(r.result, d.read("H")), r.rid
