@@ -74,10 +74,13 @@ struct af_domain {
     int max_scn = 0, max_tiles = 0;
     std::vector<af::HostScn> scn;  // max_scn slots
     std::vector<std::pair<size_t, size_t>> free_list;  // (offset, size), sorted
-    size_t o_scns = 0, o_tiles = 0, o_volbuf = 0, heap_lo = 0;
+    size_t o_scns = 0, o_tiles = 0, o_volbuf = 0, o_ctlptr = 0, o_ctlbuf = 0, heap_lo = 0;
     af::ScnDev* d_scns = nullptr;
     af::TileDesc* d_tiles = nullptr;
     double* d_volbuf = nullptr;
+    af::Ctl** d_ctlptr = nullptr;     // control blocks of the active slots, in `active` order
+    af::Ctl* d_ctlbuf = nullptr;      // their gathered copies
+    std::vector<int> active;          // active slots, ascending (as of the last tile rebuild)
     int ntiles = 0;
     long long k = 0;  // index of the next step to launch
     double last_tsum = -1.0;
@@ -290,13 +293,15 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
     return AF_OK;
 }
 
-// Table region at the start of the arena: scenario table, tile table and
-// the volume-reduction scratch.
+// Table region at the start of the arena: scenario table, tile table, the
+// volume-reduction scratch, and the control-block gather list and buffer.
 size_t plan_tables(af_domain* D, int max_scn, int max_tiles) {
     Layout L;
     D->o_scns = L.take(sizeof(ScnDev) * (size_t)max_scn);
     D->o_tiles = L.take(sizeof(TileDesc) * (size_t)max_tiles);
     D->o_volbuf = L.take(sizeof(double) * 3 * kVolBlocks);
+    D->o_ctlptr = L.take(sizeof(Ctl*) * (size_t)max_scn);
+    D->o_ctlbuf = L.take(sizeof(Ctl) * (size_t)max_scn);
     return L.top;
 }
 
@@ -625,13 +630,45 @@ int finalize_reports(af_domain* D, int s, hipStream_t st) {
     return AF_OK;
 }
 
+static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied in 8-byte words");
+constexpr int kCtlWords = (int)(sizeof(Ctl) / 8);
+
+// Copies the control blocks of the n listed scenarios into one contiguous
+// buffer, so that the host reads all of them with one copy.
+__global__ void k_gather_ctl(Ctl* const* __restrict__ list, int n, Ctl* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * kCtlWords) return;
+    const int s = i / kCtlWords, w = i - s * kCtlWords;
+    reinterpret_cast<unsigned long long*>(out)[i] =
+        reinterpret_cast<const unsigned long long*>(list[s])[w];
+}
+
+// The control blocks of the active scenarios (slots in D->active order),
+// read with one gather launch, one copy and one synchronisation.
+int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
+    const int n = (int)D->active.size();
+    out.resize(n);
+    if (n == 0) return AF_OK;
+    const int words = n * kCtlWords;
+    hipLaunchKernelGGL(k_gather_ctl, dim3((words + 255) / 256), dim3(256), 0, st, D->d_ctlptr, n,
+                       D->d_ctlbuf);
+    AF_HIP(hipGetLastError());
+    AF_HIP(hipMemcpyAsync(out.data(), D->d_ctlbuf, sizeof(Ctl) * n, hipMemcpyDeviceToHost, st));
+    AF_HIP(hipStreamSynchronize(st));
+    return AF_OK;
+}
+
 // Rebuilds and uploads the tile table from the active scenarios; a captured
 // graph holds the tile count, so it is dropped.
 int rebuild_tiles(af_domain* D, hipStream_t st) {
     std::vector<TileDesc> tiles;
+    std::vector<Ctl*> ctls;
+    std::vector<int> act;
     for (int s = 0; s < D->max_scn; s++) {
         const HostScn& h = D->scn[s];
         if (!h.active) continue;
+        act.push_back(s);
+        ctls.push_back(h.dev.ctl);
         for (int seg = 0; seg < h.nsegs; seg++)
             for (int strip = 0; strip < h.nstrips; strip++)
                 tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
@@ -657,8 +694,12 @@ int rebuild_tiles(af_domain* D, hipStream_t st) {
     if (!tiles.empty())
         AF_HIP(hipMemcpyAsync(D->d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(),
                               hipMemcpyHostToDevice, st));
+    if (!ctls.empty())
+        AF_HIP(hipMemcpyAsync(D->d_ctlptr, ctls.data(), sizeof(Ctl*) * ctls.size(),
+                              hipMemcpyHostToDevice, st));
     AF_HIP(hipStreamSynchronize(st));
     D->ntiles = (int)tiles.size();
+    D->active = std::move(act);
     if (D->gexec) {
         (void)hipGraphExecDestroy(D->gexec);
         D->gexec = nullptr;
@@ -691,6 +732,9 @@ int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn,
     D->d_scns = reinterpret_cast<ScnDev*>(D->arena + D->o_scns);
     D->d_tiles = reinterpret_cast<TileDesc*>(D->arena + D->o_tiles);
     D->d_volbuf = reinterpret_cast<double*>(D->arena + D->o_volbuf);
+    D->d_ctlptr = reinterpret_cast<Ctl**>(D->arena + D->o_ctlptr);
+    D->d_ctlbuf = reinterpret_cast<Ctl*>(D->arena + D->o_ctlbuf);
+    D->active.clear();
     return AF_OK;
 }
 
@@ -862,21 +906,19 @@ int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finish
     AF_REQUIRE(D && n_finished, AF_E_INVALID, "poll: bad arguments");
     hipStream_t st = to_stream(stream);
     const int par = (int)(D->k & 1);
-    int nf = 0, na = 0;
-    for (int s = 0; s < D->max_scn; s++) {
-        if (!D->scn[s].active) continue;
-        na++;
-        double t = 0.0;
-        AF_HIP(hipMemcpyAsync(&t, &D->scn[s].dev.ctl->t[par], sizeof(double),
-                              hipMemcpyDeviceToHost, st));
-        AF_HIP(hipStreamSynchronize(st));
-        if (!(t < D->scn[s].dev.sim_time)) {
+    std::vector<Ctl> c;
+    int rc = gather_ctl(D, c, st);
+    if (rc) return rc;
+    int nf = 0;
+    for (size_t a = 0; a < D->active.size(); a++) {
+        const int s = D->active[a];
+        if (!(c[a].t[par] < D->scn[s].dev.sim_time)) {
             if (finished && nf < cap) finished[nf] = s;
             nf++;
         }
     }
     *n_finished = nf;
-    if (n_active) *n_active = na;
+    if (n_active) *n_active = (int)D->active.size();
     return AF_OK;
 }
 
@@ -926,18 +968,13 @@ int af_domain_advance(af_domain* D, int64_t max_steps, int use_graph, af_stream_
 int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
     AF_REQUIRE(D && all_done, AF_E_INVALID, "finished: bad arguments");
     hipStream_t st = to_stream(stream);
-    const int n = D->max_scn;
-    std::vector<Ctl> c(n);
-    for (int s = 0; s < n; s++)
-        if (D->scn[s].active)
-            AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost,
-                                  st));
-    AF_HIP(hipStreamSynchronize(st));
+    std::vector<Ctl> c;
+    int rc = gather_ctl(D, c, st);
+    if (rc) return rc;
+    const int par = (int)(D->k & 1);
     int done = 1;
-    for (int s = 0; s < n; s++) {
-        const int par = (int)(D->k & 1);
-        if (D->scn[s].active && c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
-    }
+    for (size_t a = 0; a < D->active.size(); a++)
+        if (c[a].t[par] < D->scn[D->active[a]].dev.sim_time) done = 0;
     *all_done = done;
     return AF_OK;
 }
@@ -945,24 +982,19 @@ int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
 int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t stream) {
     AF_REQUIRE(D && check_every > 0, AF_E_INVALID, "run: bad arguments");
     hipStream_t st = to_stream(stream);
-    const int n = D->max_scn;
-    std::vector<Ctl> c(n);
+    std::vector<Ctl> c;
     long long prev_done = -1;
     for (;;) {
-        for (int s = 0; s < n; s++)
-            if (D->scn[s].active)
-                AF_HIP(hipMemcpyAsync(&c[s], D->scn[s].dev.ctl, sizeof(Ctl),
-                                      hipMemcpyDeviceToHost, st));
-        AF_HIP(hipStreamSynchronize(st));
+        int rc = gather_ctl(D, c, st);
+        if (rc) return rc;
         const int par = (int)(D->k & 1);
         int done = 1;
         long long steps = 0;
         double tsum = 0.0;
-        for (int s = 0; s < n; s++) {
-            if (!D->scn[s].active) continue;
-            if (c[s].t[par] < D->scn[s].dev.sim_time) done = 0;
-            steps += c[s].n_done;
-            tsum += c[s].t[par];
+        for (size_t a = 0; a < D->active.size(); a++) {
+            if (c[a].t[par] < D->scn[D->active[a]].dev.sim_time) done = 0;
+            steps += c[a].n_done;
+            tsum += c[a].t[par];
         }
         if (done) break;
         // a clock that no longer advances (dt == 0 after an overflow) would never end
@@ -972,7 +1004,7 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
         }
         prev_done = steps;
         D->last_tsum = tsum;
-        int rc = af_domain_advance(D, check_every, use_graph, stream);
+        rc = af_domain_advance(D, check_every, use_graph, stream);
         if (rc) return rc;
     }
     return AF_OK;
