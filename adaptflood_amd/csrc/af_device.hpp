@@ -165,43 +165,32 @@ __device__ __forceinline__ void face_depths_f32(float z0, float z1, float h0, fl
 //   g*dt*hflow*Sf          = -(g*dt/dx)*hflow*dh
 //   g*dt*hflow*n^2*|q| / hflow^(10/3) = g*dt*n^2*|q| * hflow^(-7/3)
 // with hflow^(-7/3) = exp2(-7/3*log2(hflow)) on the hardware transcendental
-// units (hflow > DepthThresh > 0 here) and one reciprocal for num/den.
+// units and one reciprocal for num/den.  The wet-face formula is evaluated on
+// every lane and the result selected by hflow > DepthThresh: a divergent
+// branch per face costs more exec-mask instructions than the formula, and a
+// lone wavefront's row is bound by the instructions it issues.  (On a dry
+// lane the discarded value may be Inf or NaN; it is never used.)
 __device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, float h1, float q0,
                                                float qt, const FluxConstsF32& k) {
     float dh, hflow;
     face_depths_f32(z0, z1, h0, h1, dh, hflow);
     hflow = getmax(hflow, 0.0f);
     hflow = getmin(hflow, k.maxh);
-    float Q = 0.0f;
-    if (hflow > k.thresh) {
-        float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
-        float num = q0 + k.gdt_dx * hflow * dh;
-        float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
-        float den = 1.0f + k.gdt_n2 * qvect * p;
-        Q = num * k.dx * __builtin_amdgcn_rcpf(den);
-    }
-    return Q;
+    const float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
+    const float num = q0 + k.gdt_dx * hflow * dh;
+    const float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
+    const float den = 1.0f + k.gdt_n2 * qvect * p;
+    const float Q = num * k.dx * __builtin_amdgcn_rcpf(den);
+    return (hflow > k.thresh) ? Q : 0.0f;
 }
 
 // face_flux_f32 on stored discharges (see face_flux_ref_t1_q).
 __device__ __forceinline__ float face_flux_f32_q(float z0, float z1, float h0, float h1, float Q0,
                                                  float a, float b, float c, float d,
                                                  const FluxConstsF32& k) {
-    float dh, hflow;
-    face_depths_f32(z0, z1, h0, h1, dh, hflow);
-    hflow = getmax(hflow, 0.0f);
-    hflow = getmin(hflow, k.maxh);
-    float Q = 0.0f;
-    if (hflow > k.thresh) {
-        const float q0 = Q0 * k.sc;
-        const float qt = (((a + b) + c) + d) * k.sc4;
-        float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
-        float num = q0 + k.gdt_dx * hflow * dh;
-        float p = __builtin_amdgcn_exp2f(-2.33333333f * __builtin_amdgcn_logf(hflow));
-        float den = 1.0f + k.gdt_n2 * qvect * p;
-        Q = num * k.dx * __builtin_amdgcn_rcpf(den);
-    }
-    return Q;
+    const float q0 = Q0 * k.sc;
+    const float qt = (((a + b) + c) + d) * k.sc4;
+    return face_flux_f32(z0, z1, h0, h1, q0, qt, k);
 }
 
 // Edge fluxes of src/kernels.cpp:154-253 from the pre-update depth.

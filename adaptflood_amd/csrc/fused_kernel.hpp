@@ -220,11 +220,13 @@ struct Stepper {
     static __device__ __forceinline__ T flux(T z0, T z1, T h0, T h1, T Q0, T a, T b, T c, T d,
                                              T nod, T sc, const FluxConstsF64& kd,
                                              const FluxConstsF32& kf) {
-        if (z0 == nod || z1 == nod) return T(0);
         if constexpr (F64) {
+            if (z0 == nod || z1 == nod) return T(0);
             return face_flux_ref_t1_q(z0, z1, h0, h1, Q0, a, b, c, d, sc, kd);
         } else {
-            return face_flux_f32_q(z0, z1, h0, h1, Q0, a, b, c, d, kf);
+            // no branch (see face_flux_f32)
+            const T q = face_flux_f32_q(z0, z1, h0, h1, Q0, a, b, c, d, kf);
+            return (z0 == nod || z1 == nod) ? T(0) : q;
         }
     }
 
@@ -283,9 +285,11 @@ struct Stepper {
             qxw[v] = q;
         }
         // east face of the lane's last column: the next lane's west face,
-        // or for lane 63 the strip's east face
+        // or for lane 63 the strip's east face.  Every lane evaluates the
+        // strip's east face (a face evaluated for one lane occupies the
+        // wavefront as long as one for all) and lane 63 keeps it.
         T qxe_last = __shfl_down(qxw[0], 1, 64);
-        if (lane == 63) {
+        {
             const int f = x.i0 + W;
             T q = T(0);
             if (!EDGE || f < nx) {
@@ -295,7 +299,7 @@ struct Stepper {
                 q = S.east_free ? edge_out_pos<T>(c.h[V - 1], c.z[V - 1] == nod, x.dxT, x.gT)
                                 : T(0);
             }
-            qxe_last = q;
+            if (lane == 63) qxe_last = q;
         }
 
         // y faces south of row j (face row j+1)
