@@ -66,6 +66,8 @@ struct af_domain {
     int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36, variant = 0;
     int R_fixed = 0;             // tile rows asked for (0: chosen per scenario)
     int tile_order = 0;          // af_tuning.tile_order
+    int min_rows = 8;            // lowest default tile height (see kMinRowsAlone)
+    bool pool = false;           // created by af_pool_create
     long long dt_log_cap = 0;
     int report_cap = 64;
     char* arena = nullptr;
@@ -93,6 +95,13 @@ namespace {
 
 constexpr int kVolBlocks = 1024;
 constexpr int kMinTiles = 2048;   // wavefronts a lone scenario should offer
+// Lowest default tile height: a scenario that runs by itself (a lone domain,
+// or one admitted to a pool holding fewer than kMinTiles tiles) goes down to
+// kMinRowsAlone rows; one that shares the launch with enough other tiles
+// stops at kMinRowsShared, where the re-read row above and below each segment
+// costs a quarter more reads instead of half.
+constexpr int kMinRowsAlone = 4;
+constexpr int kMinRowsShared = 8;
 
 int elem_size(int precision) { return precision == AF_FP64 ? 8 : 4; }
 
@@ -162,13 +171,15 @@ void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     h.P = (long long)(row / es);
     h.nstrips = (d->nx + D->W - 1) / D->W;
     // Tile rows: the tuning's value, else the default height, halved while
-    // the scenario alone would give fewer than kMinTiles wavefronts (a small
-    // domain run by itself is latency-bound: short tiles keep more rows in
-    // flight at once).
+    // the scenario alone would give fewer than kMinTiles wavefronts, down to
+    // D->min_rows.  A small domain run by itself is latency-bound: each
+    // wavefront marches its rows one after another, so a step takes as long
+    // as the tallest tile, and shorter tiles split the rows over more
+    // wavefronts.
     int R = D->R_fixed;
     if (R <= 0) {
         R = D->R;
-        while (R > 8 && (long long)h.nstrips * ((d->ny + R - 1) / R) < kMinTiles) R /= 2;
+        while (R > D->min_rows && (long long)h.nstrips * ((d->ny + R - 1) / R) < kMinTiles) R /= 2;
     }
     h.R = R;
     h.nsegs = (d->ny + R - 1) / R;
@@ -747,10 +758,11 @@ int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool reb
     for (int i = 0; i < D->max_scn; i++)
         if (!D->scn[i].active) { s = i; break; }
     AF_REQUIRE(s >= 0, AF_E_CAPACITY, "no free scenario slot (%d in use)", D->max_scn);
-    HostScn h;
-    const size_t bytes = scenario_bytes(D, d, h);
     int used = 0;
     for (const HostScn& o : D->scn) used += o.active ? o.ntiles : 0;
+    if (D->pool) D->min_rows = (used < kMinTiles) ? kMinRowsAlone : kMinRowsShared;
+    HostScn h;
+    const size_t bytes = scenario_bytes(D, d, h);
     AF_REQUIRE(used + h.ntiles <= D->max_tiles, AF_E_CAPACITY,
                "tile table full: %d in use, %d more needed, %d max", used, h.ntiles,
                D->max_tiles);
@@ -793,6 +805,7 @@ int af_batch_arena_bytes(const af_scenario_desc* descs, int n, int precision,
     af_domain tmp;
     int rc = configure(&tmp, precision, tuning);
     if (rc) return rc;
+    tmp.min_rows = (n == 1) ? kMinRowsAlone : kMinRowsShared;
     size_t heap = 0;
     int tiles = 0;
     for (int s = 0; s < n; s++) {
@@ -821,6 +834,7 @@ int af_batch_create(const af_scenario_desc* descs, int n, int precision, const a
                need, arena_bytes);
     af_domain tmp;
     (void)configure(&tmp, precision, tuning);
+    tmp.min_rows = (n == 1) ? kMinRowsAlone : kMinRowsShared;
     int tiles = 0;
     for (int s = 0; s < n; s++) {
         HostScn h;
@@ -830,6 +844,7 @@ int af_batch_create(const af_scenario_desc* descs, int n, int precision, const a
     af_domain* D = new af_domain();
     hipStream_t st = to_stream(stream);
     rc = pool_init(D, precision, tuning, n, tiles, arena, arena_bytes);
+    D->min_rows = tmp.min_rows;
     for (int s = 0; s < n && rc == AF_OK; s++) {
         int slot = -1;
         rc = pool_admit(D, &descs[s], st, false, &slot);
@@ -864,6 +879,7 @@ int af_pool_scenario_bytes(const af_scenario_desc* desc, int precision, const af
     if (rc) return rc;
     rc = validate(desc);
     if (rc) return rc;
+    tmp.min_rows = kMinRowsAlone;  // the most tiles an admission can choose
     HostScn h;
     *bytes = scenario_bytes(&tmp, desc, h);
     if (tiles) *tiles = h.ntiles;
@@ -875,6 +891,7 @@ int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, in
     AF_REQUIRE(out, AF_E_INVALID, "pool_create: null out");
     af_domain* D = new af_domain();
     int rc = pool_init(D, precision, tuning, max_scenarios, max_tiles, arena, arena_bytes);
+    D->pool = true;
     if (rc) {
         delete D;
         return rc;

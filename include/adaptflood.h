@@ -181,8 +181,10 @@ typedef struct af_domain af_domain; /* opaque, host side */
 /* Tile shape: each wavefront owns a strip of 64*V columns (V = 4 at FP32,
  * 2 at FP64) and marches down `tile_rows` rows.  0 = library default: 64 rows
  * at FP32 and 32 at FP64, halved per scenario while the scenario would offer
- * fewer than 2048 wavefronts, down to 8 rows.  No field changes any result:
- * the tile shape, variant and order only change how the work is split. */
+ * fewer than 2048 wavefronts, down to 4 rows for a scenario that runs alone
+ * (a domain, a batch of one, or an admission to a pool holding fewer than
+ * 2048 tiles) and to 8 rows otherwise.  No field changes any result: the
+ * tile shape, variant and order only change how the work is split. */
 typedef struct af_tuning {
     int32_t tile_rows;
     int32_t waves_per_block; /* 1 or 4 */
@@ -275,7 +277,9 @@ int af_batch_launch_step(af_domain* d, af_stream_t stream);
  * every af_domain_* / af_batch_* call applies to it, with slot indices as
  * scenario indices.  Admission and release take effect between launches. */
 int af_pool_arena_bytes(int max_scenarios, int max_tiles, size_t heap_bytes, size_t* bytes);
-/* Heap bytes and tile-table entries one scenario needs (its HBM footprint). */
+/* Heap bytes and tile-table entries one scenario needs (its HBM footprint):
+ * with the default tiling, the most an admission can take (its shortest
+ * tiles). */
 int af_pool_scenario_bytes(const af_scenario_desc* desc, int precision, const af_tuning* tuning,
                            size_t* bytes, int32_t* tiles);
 int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, int max_tiles,

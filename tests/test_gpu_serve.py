@@ -112,3 +112,23 @@ def test_request_over_budget_is_rejected(gpu):
         d = Domain(r.scenario, precision="fp32")
         d.run(check_every=12, graph=True)
         assert np.array_equal(r.result, d.read("H")), r.rid
+
+
+def test_default_tile_height_follows_pool_load(gpu):
+    """Default tiling in a pool: a small scenario admitted to an empty pool
+    takes the shortest tiles (4 rows: it runs alone and is latency-bound);
+    admitted beside a domain that already offers 2048 tiles, it stops at 8
+    rows.  The footprint query reports the larger, 4-row tile count."""
+    from adaptflood_amd.scenario import Scenario
+    small = generators.small_case(128, 32, seed=140, sim_time=30.0)
+    dem = np.tile(np.linspace(2.0, 0.0, 2048), (2048, 1))
+    big = Scenario(dem=dem, dx=5.0, sim_time=10.0)
+    alone = Pool(budget_bytes=64 << 20, precision="fp32", max_scenarios=2, max_tiles=8192)
+    assert alone.footprint(small)[1] == 8              # 1 strip x 32/4 segments
+    alone.admit(small)
+    assert alone.stats()["tiles"] == 8
+    shared = Pool(budget_bytes=1 << 30, precision="fp32", max_scenarios=2, max_tiles=8192)
+    shared.admit(big)
+    assert shared.stats()["tiles"] == 2048             # 8 strips x 2048/8 segments
+    shared.admit(small)
+    assert shared.stats()["tiles"] == 2048 + 4         # 1 strip x 32/8 segments
