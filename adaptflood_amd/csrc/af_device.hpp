@@ -263,6 +263,36 @@ __host__ __device__ __forceinline__ double interp_series(PD times, PD values, in
     return values[n - 1];
 }
 
+// interp_series with every load of a short series (up to four points, as a
+// hydrograph or a rain record usually is) issued at once, so that a value
+// costs one round trip to memory instead of one per bracket visited.  The
+// indices are clamped to the last point, which makes the padded brackets
+// empty; the end tests and the brackets are visited in interp_series' order
+// with its arithmetic, so the value is the same for every input.
+template <typename PD>
+__host__ __device__ __forceinline__ double interp_series_short(PD times, PD values, int n,
+                                                      double empty_value, double t) {
+    if (n <= 0) return empty_value;
+    if (n > 4) return interp_series(times, values, n, empty_value, t);
+    double tt[4], vv[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int c = (i < n) ? i : n - 1;
+        tt[i] = times[c];
+        vv[i] = values[c];
+    }
+    if (t <= tt[0]) return vv[0];
+    if (t >= tt[3]) return vv[3];  // tt[3] is times[n-1]
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (t >= tt[i] && t < tt[i + 1]) {
+            double ratio = (t - tt[i]) / (tt[i + 1] - tt[i]);
+            return vv[i] + ratio * (vv[i + 1] - vv[i]);
+        }
+    }
+    return vv[3];
+}
+
 // dt of src/kernels.cpp:36-43
 __host__ __device__ __forceinline__ double cfl_dt(double hmax, double thresh, double cfl,
                                                   double dx, double g, double init_tstep) {

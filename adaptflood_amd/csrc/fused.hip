@@ -42,14 +42,14 @@ struct HostScn {
     double zmin = 0.0;
     // offsets in the arena
     size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
-    size_t o_src_begin, o_src_j, o_src_i, o_src_pt;
-    size_t o_ovr_begin, o_ovr_j, o_ovr_i, o_ovr_pt;
-    size_t o_pt_type, o_pt_value, o_pt_off, o_pt_len, o_series_t, o_series_v;
+    size_t o_src_begin, o_src_j, o_src_i, o_src_val, o_src_off, o_src_len;
+    size_t o_ovr_begin, o_ovr_j, o_ovr_i, o_ovr_val, o_ovr_off, o_ovr_len;
+    size_t o_series_t, o_series_v;
     size_t o_rain_t, o_rain_r, o_ctl, o_part_vol, o_part_comp, o_part_area, o_dt_log, o_reports;
     // host-built plan contents
-    std::vector<int> src_begin, src_j, src_i, src_pt, ovr_begin, ovr_j, ovr_i, ovr_pt;
-    std::vector<int> pt_type, pt_off, pt_len;
-    std::vector<double> pt_value;
+    std::vector<int> src_begin, src_j, src_i, src_off, src_len;
+    std::vector<int> ovr_begin, ovr_j, ovr_i, ovr_off, ovr_len;
+    std::vector<double> src_val, ovr_val;
     int n_series = 0, n_rain = 0;
     ScnDev dev{};
     // pool bookkeeping
@@ -144,9 +144,9 @@ int validate(const af_scenario_desc* d) {
 // Adds the heap base of the scenario's extent to every offset.
 void rebase(HostScn& h, size_t b) {
     size_t* fields[] = {&h.o_z, &h.o_H[0], &h.o_H[1], &h.o_Qx[0], &h.o_Qx[1], &h.o_Qy[0],
-                        &h.o_Qy[1], &h.o_src_begin, &h.o_src_j, &h.o_src_i, &h.o_src_pt,
-                        &h.o_ovr_begin, &h.o_ovr_j, &h.o_ovr_i, &h.o_ovr_pt, &h.o_pt_type,
-                        &h.o_pt_value, &h.o_pt_off, &h.o_pt_len, &h.o_series_t, &h.o_series_v,
+                        &h.o_Qy[1], &h.o_src_begin, &h.o_src_j, &h.o_src_i, &h.o_src_val,
+                        &h.o_src_off, &h.o_src_len, &h.o_ovr_begin, &h.o_ovr_j, &h.o_ovr_i,
+                        &h.o_ovr_val, &h.o_ovr_off, &h.o_ovr_len, &h.o_series_t, &h.o_series_v,
                         &h.o_rain_t, &h.o_rain_r, &h.o_ctl, &h.o_part_vol, &h.o_part_comp,
                         &h.o_part_area, &h.o_dt_log, &h.o_reports};
     for (size_t* f : fields) *f += b;
@@ -205,16 +205,15 @@ void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layo
     h.o_src_begin = L.take(nt1 * 4);
     h.o_src_j = L.take(n_src * 4);
     h.o_src_i = L.take(n_src * 4);
-    h.o_src_pt = L.take(n_src * 4);
+    h.o_src_val = L.take(n_src * 8);
+    h.o_src_off = L.take(n_src * 4);
+    h.o_src_len = L.take(n_src * 4);
     h.o_ovr_begin = L.take(nt1 * 4);
     h.o_ovr_j = L.take(n_ovr * 4);
     h.o_ovr_i = L.take(n_ovr * 4);
-    h.o_ovr_pt = L.take(n_ovr * 4);
-    const size_t np = (size_t)d->n_points;
-    h.o_pt_type = L.take(np * 4);
-    h.o_pt_value = L.take(np * 8);
-    h.o_pt_off = L.take(np * 4);
-    h.o_pt_len = L.take(np * 4);
+    h.o_ovr_val = L.take(n_ovr * 8);
+    h.o_ovr_off = L.take(n_ovr * 4);
+    h.o_ovr_len = L.take(n_ovr * 4);
     h.o_series_t = L.take((size_t)d->n_series * 8);
     h.o_series_v = L.take((size_t)d->n_series * 8);
     h.o_rain_t = L.take((size_t)(d->rain_enabled ? d->n_rain : 0) * 8);
@@ -229,7 +228,11 @@ void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layo
 
 // Points sorted per tile by (row, column, file order): a stable sort keeps
 // file order among points of one cell, which is the order the reference's
-// sequential loops would apply them in.
+// sequential loops would apply them in.  Each entry carries what its value
+// needs: a constant (QFIX and HFIX value, 0 for FREE) with an empty series,
+// or a series (QVAR, HVAR) whose empty-series value is the point's value.
+// Interpolating an empty series returns that value, so every entry is
+// evaluated the same way, from the entry itself (main.cpp:245-253, 275-279).
 void plan_points(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     struct Item { int tile, j, i, pt; };
     std::vector<Item> src, ovr;
@@ -249,23 +252,24 @@ void plan_points(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     std::stable_sort(src.begin(), src.end(), key_less);
     std::stable_sort(ovr.begin(), ovr.end(), key_less);
     auto build = [&](const std::vector<Item>& v, std::vector<int>& begin, std::vector<int>& J,
-                     std::vector<int>& I, std::vector<int>& PT) {
+                     std::vector<int>& I, std::vector<double>& VAL, std::vector<int>& OFF,
+                     std::vector<int>& LEN) {
         begin.assign(h.ntiles + 1, 0);
         for (const Item& it : v) begin[it.tile + 1]++;
         for (int t = 0; t < h.ntiles; t++) begin[t + 1] += begin[t];
-        J.clear(); I.clear(); PT.clear();
-        for (const Item& it : v) { J.push_back(it.j); I.push_back(it.i); PT.push_back(it.pt); }
+        J.clear(); I.clear(); VAL.clear(); OFF.clear(); LEN.clear();
+        for (const Item& it : v) {
+            const int p = it.pt, ty = d->pt_type[p];
+            const bool series = (ty == AF_BC_QVAR || ty == AF_BC_HVAR);
+            J.push_back(it.j);
+            I.push_back(it.i);
+            VAL.push_back(ty == AF_BC_FREE ? 0.0 : d->pt_value[p]);
+            OFF.push_back(series && d->pt_series_off ? d->pt_series_off[p] : 0);
+            LEN.push_back(series && d->pt_series_len ? d->pt_series_len[p] : 0);
+        }
     };
-    build(src, h.src_begin, h.src_j, h.src_i, h.src_pt);
-    build(ovr, h.ovr_begin, h.ovr_j, h.ovr_i, h.ovr_pt);
-    h.pt_type.assign(d->pt_type, d->pt_type + d->n_points);
-    h.pt_value.assign(d->pt_value, d->pt_value + d->n_points);
-    h.pt_off.assign(d->n_points, 0);
-    h.pt_len.assign(d->n_points, 0);
-    for (int p = 0; p < d->n_points; p++) {
-        if (d->pt_series_off) h.pt_off[p] = d->pt_series_off[p];
-        if (d->pt_series_len) h.pt_len[p] = d->pt_series_len[p];
-    }
+    build(src, h.src_begin, h.src_j, h.src_i, h.src_val, h.src_off, h.src_len);
+    build(ovr, h.ovr_begin, h.ovr_j, h.ovr_i, h.ovr_val, h.ovr_off, h.ovr_len);
     h.n_series = d->n_series;
     h.n_rain = d->rain_enabled ? d->n_rain : 0;
 }
@@ -437,15 +441,15 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     v.src_begin = reinterpret_cast<const int*>(A + h.o_src_begin);
     v.src_j = reinterpret_cast<const int*>(A + h.o_src_j);
     v.src_i = reinterpret_cast<const int*>(A + h.o_src_i);
-    v.src_pt = reinterpret_cast<const int*>(A + h.o_src_pt);
+    v.src_val = reinterpret_cast<const double*>(A + h.o_src_val);
+    v.src_off = reinterpret_cast<const int*>(A + h.o_src_off);
+    v.src_len = reinterpret_cast<const int*>(A + h.o_src_len);
     v.ovr_begin = reinterpret_cast<const int*>(A + h.o_ovr_begin);
     v.ovr_j = reinterpret_cast<const int*>(A + h.o_ovr_j);
     v.ovr_i = reinterpret_cast<const int*>(A + h.o_ovr_i);
-    v.ovr_pt = reinterpret_cast<const int*>(A + h.o_ovr_pt);
-    v.pt_type = reinterpret_cast<const int*>(A + h.o_pt_type);
-    v.pt_value = reinterpret_cast<const double*>(A + h.o_pt_value);
-    v.pt_off = reinterpret_cast<const int*>(A + h.o_pt_off);
-    v.pt_len = reinterpret_cast<const int*>(A + h.o_pt_len);
+    v.ovr_val = reinterpret_cast<const double*>(A + h.o_ovr_val);
+    v.ovr_off = reinterpret_cast<const int*>(A + h.o_ovr_off);
+    v.ovr_len = reinterpret_cast<const int*>(A + h.o_ovr_len);
     v.series_t = reinterpret_cast<const double*>(A + h.o_series_t);
     v.series_v = reinterpret_cast<const double*>(A + h.o_series_v);
     v.rain_enabled = d->rain_enabled ? 1 : 0;
@@ -493,15 +497,15 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     if ((rc = up(h.o_src_begin, h.src_begin.data(), h.src_begin.size() * 4))) return rc;
     if ((rc = up(h.o_src_j, h.src_j.data(), h.src_j.size() * 4))) return rc;
     if ((rc = up(h.o_src_i, h.src_i.data(), h.src_i.size() * 4))) return rc;
-    if ((rc = up(h.o_src_pt, h.src_pt.data(), h.src_pt.size() * 4))) return rc;
+    if ((rc = up(h.o_src_val, h.src_val.data(), h.src_val.size() * 8))) return rc;
+    if ((rc = up(h.o_src_off, h.src_off.data(), h.src_off.size() * 4))) return rc;
+    if ((rc = up(h.o_src_len, h.src_len.data(), h.src_len.size() * 4))) return rc;
     if ((rc = up(h.o_ovr_begin, h.ovr_begin.data(), h.ovr_begin.size() * 4))) return rc;
     if ((rc = up(h.o_ovr_j, h.ovr_j.data(), h.ovr_j.size() * 4))) return rc;
     if ((rc = up(h.o_ovr_i, h.ovr_i.data(), h.ovr_i.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_pt, h.ovr_pt.data(), h.ovr_pt.size() * 4))) return rc;
-    if ((rc = up(h.o_pt_type, h.pt_type.data(), h.pt_type.size() * 4))) return rc;
-    if ((rc = up(h.o_pt_value, h.pt_value.data(), h.pt_value.size() * 8))) return rc;
-    if ((rc = up(h.o_pt_off, h.pt_off.data(), h.pt_off.size() * 4))) return rc;
-    if ((rc = up(h.o_pt_len, h.pt_len.data(), h.pt_len.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_val, h.ovr_val.data(), h.ovr_val.size() * 8))) return rc;
+    if ((rc = up(h.o_ovr_off, h.ovr_off.data(), h.ovr_off.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_len, h.ovr_len.data(), h.ovr_len.size() * 4))) return rc;
     if ((rc = up(h.o_series_t, d->series_t, (size_t)h.n_series * 8))) return rc;
     if ((rc = up(h.o_series_v, d->series_v, (size_t)h.n_series * 8))) return rc;
     if ((rc = up(h.o_rain_t, d->rain_t, (size_t)h.n_rain * 8))) return rc;

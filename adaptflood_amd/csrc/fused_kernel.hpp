@@ -58,19 +58,21 @@ struct ScnDev {
     double nodata_t;              // sentinel compared against z (as T)
     int west_free, east_free, north_free, south_free;
     // point sources (QFIX/QVAR) and overrides (FREE/HFIX/HVAR), sorted per
-    // tile by (row, column, file order); *_begin has ntiles+1 entries
+    // tile by (row, column, file order); *_begin has ntiles+1 entries.  An
+    // entry's value is its series [off, off+len) at the step's start time,
+    // or `val` when the series is empty (constants have len = 0).
     const int* src_begin;
     const int* src_j;
     const int* src_i;
-    const int* src_pt;
+    const double* src_val;
+    const int* src_off;
+    const int* src_len;
     const int* ovr_begin;
     const int* ovr_j;
     const int* ovr_i;
-    const int* ovr_pt;
-    const int* pt_type;
-    const double* pt_value;
-    const int* pt_off;
-    const int* pt_len;
+    const double* ovr_val;
+    const int* ovr_off;
+    const int* ovr_len;
     const double* series_t;
     const double* series_v;
     int rain_enabled, n_rain;
@@ -421,17 +423,11 @@ struct Stepper {
     // Value of source entry k (the list is sorted per tile): v at the step's
     // start time, then ps = v*dx*dt/dA (main.cpp:246-253) as the depth added.
     static __device__ __forceinline__ T source_depth(const ScnDev& S, const TileCtx<T>& x, int k) {
-        const int pt = glob<const int>(S.src_pt)[k];
-        gptr<const double> pt_value = glob<const double>(S.pt_value);
-        double val;
-        if (glob<const int>(S.pt_type)[pt] == 4) {
-            val = pt_value[pt];
-        } else {
-            const int off = glob<const int>(S.pt_off)[pt];
-            val = interp_series(glob<const double>(S.series_t) + off,
-                                glob<const double>(S.series_v) + off,
-                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
-        }
+        const int off = glob<const int>(S.src_off)[k];
+        const double val = interp_series_short(glob<const double>(S.series_t) + off,
+                                               glob<const double>(S.series_v) + off,
+                                               glob<const int>(S.src_len)[k],
+                                               glob<const double>(S.src_val)[k], x.t);
         const double ps = val * S.dx * x.dt / x.dA;
         return (T)ps;
     }
@@ -441,18 +437,11 @@ struct Stepper {
     // main.cpp:275-279.
     static __device__ __forceinline__ T override_depth(const ScnDev& S, const TileCtx<T>& x,
                                                        int k) {
-        const int pt = glob<const int>(S.ovr_pt)[k];
-        const int ty = glob<const int>(S.pt_type)[pt];
-        gptr<const double> pt_value = glob<const double>(S.pt_value);
-        double val = 0.0;
-        if (ty == 2) {
-            val = pt_value[pt];
-        } else if (ty == 3) {
-            const int off = glob<const int>(S.pt_off)[pt];
-            val = interp_series(glob<const double>(S.series_t) + off,
-                                glob<const double>(S.series_v) + off,
-                                glob<const int>(S.pt_len)[pt], pt_value[pt], x.t);
-        }
+        const int off = glob<const int>(S.ovr_off)[k];
+        const double val = interp_series_short(glob<const double>(S.series_t) + off,
+                                               glob<const double>(S.series_v) + off,
+                                               glob<const int>(S.ovr_len)[k],
+                                               glob<const double>(S.ovr_val)[k], x.t);
         return (T)val;
     }
 
@@ -613,8 +602,8 @@ struct Stepper {
         double rain_depth = 0.0;
         x.rain_on = false;
         if (S.rain_enabled) {
-            double r = interp_series(glob<const double>(S.rain_t), glob<const double>(S.rain_r),
-                                     S.n_rain, 0.0, t);
+            double r = interp_series_short(glob<const double>(S.rain_t),
+                                           glob<const double>(S.rain_r), S.n_rain, 0.0, t);
             x.rain_on = (r > 0.0);
             if (x.rain_on) rain_depth = r * dt;
         }
