@@ -39,6 +39,8 @@ struct HostScn {
     long long P = 0;
     int nstrips = 0, nsegs = 0, ntiles = 0, tile_base = 0;
     int R = 0;                   // tile rows of this scenario
+    int R_alone = 0;             // tile rows it would take running alone (<= R)
+    int cap_tiles = 0;           // tiles at R_alone: the per-tile arrays are sized for it
     double zmin = 0.0;
     // offsets in the arena
     size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
@@ -176,14 +178,21 @@ void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     // wavefront marches its rows one after another, so a step takes as long
     // as the tallest tile, and shorter tiles split the rows over more
     // wavefronts.
-    int R = D->R_fixed;
+    auto halve = [&](int floor_rows) {
+        int r = D->R;
+        while (r > floor_rows && (long long)h.nstrips * ((d->ny + r - 1) / r) < kMinTiles) r /= 2;
+        return r;
+    };
+    int R = D->R_fixed, Ra = D->R_fixed;
     if (R <= 0) {
-        R = D->R;
-        while (R > D->min_rows && (long long)h.nstrips * ((d->ny + R - 1) / R) < kMinTiles) R /= 2;
+        R = halve(D->min_rows);
+        Ra = halve(kMinRowsAlone);
     }
     h.R = R;
+    h.R_alone = Ra;
     h.nsegs = (d->ny + R - 1) / R;
     h.ntiles = h.nstrips * h.nsegs;
+    h.cap_tiles = h.nstrips * ((d->ny + Ra - 1) / Ra);
 }
 
 void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layout& L) {
@@ -201,7 +210,7 @@ void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layo
         if (ty == AF_BC_QFIX || ty == AF_BC_QVAR) n_src++;
         else if (ty == AF_BC_FREE || ty == AF_BC_HFIX || ty == AF_BC_HVAR) n_ovr++;
     }
-    const size_t nt1 = (size_t)h.ntiles + 1;
+    const size_t nt1 = (size_t)h.cap_tiles + 1;
     h.o_src_begin = L.take(nt1 * 4);
     h.o_src_j = L.take(n_src * 4);
     h.o_src_i = L.take(n_src * 4);
@@ -219,9 +228,9 @@ void plan_layout(HostScn& h, const af_scenario_desc* d, const af_domain* D, Layo
     h.o_rain_t = L.take((size_t)(d->rain_enabled ? d->n_rain : 0) * 8);
     h.o_rain_r = L.take((size_t)(d->rain_enabled ? d->n_rain : 0) * 8);
     h.o_ctl = L.take(sizeof(Ctl));
-    h.o_part_vol = L.take(2 * (size_t)h.ntiles * 8);
-    h.o_part_comp = L.take(2 * (size_t)h.ntiles * 8);
-    h.o_part_area = L.take(2 * (size_t)h.ntiles * 8);
+    h.o_part_vol = L.take(2 * (size_t)h.cap_tiles * 8);
+    h.o_part_comp = L.take(2 * (size_t)h.cap_tiles * 8);
+    h.o_part_area = L.take(2 * (size_t)h.cap_tiles * 8);
     h.o_dt_log = L.take((size_t)D->dt_log_cap * 8);
     h.o_reports = L.take((size_t)D->report_cap * 32);
 }
@@ -726,6 +735,97 @@ int rebuild_tiles(af_domain* D, hipStream_t st) {
     return AF_OK;
 }
 
+// Re-sorts one point list (sources or overrides) of scenario h for its
+// present tile height: by (tile, row, column), stably, so entries of one cell
+// keep their file order.
+void resort_points(const HostScn& h, int W, std::vector<int>& begin, std::vector<int>& J,
+                   std::vector<int>& I, std::vector<double>& VAL, std::vector<int>& OFF,
+                   std::vector<int>& LEN) {
+    const size_t n = J.size();
+    std::vector<int> tile(n), idx(n);
+    for (size_t e = 0; e < n; e++) {
+        tile[e] = (I[e] / W) + (J[e] / h.R) * h.nstrips;
+        idx[e] = (int)e;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        if (tile[a] != tile[b]) return tile[a] < tile[b];
+        if (J[a] != J[b]) return J[a] < J[b];
+        return I[a] < I[b];
+    });
+    auto permute = [&](auto& v) {
+        auto w = v;
+        for (size_t e = 0; e < n; e++) w[e] = v[idx[e]];
+        v.swap(w);
+    };
+    permute(J);
+    permute(I);
+    permute(VAL);
+    permute(OFF);
+    permute(LEN);
+    begin.assign(h.ntiles + 1, 0);
+    for (size_t e = 0; e < n; e++) begin[tile[idx[e]] + 1]++;
+    for (int t = 0; t < h.ntiles; t++) begin[t + 1] += begin[t];
+}
+
+// Moves active scenario s to tile height R (R_alone <= R): completes a mass
+// report pending from the last step (its partials are laid out by the old
+// tiles), re-sorts and uploads the point lists, and uploads the scenario's
+// constants.  Results do not depend on the tiling.  The caller rebuilds the
+// tile table.
+int retile(af_domain* D, int s, int R, hipStream_t st) {
+    HostScn& h = D->scn[s];
+    hipLaunchKernelGGL(k_finalize_reports, dim3(1), dim3(64), 0, st, D->d_scns + s);
+    AF_HIP(hipGetLastError());
+    h.R = R;
+    h.nsegs = (h.ny + R - 1) / R;
+    h.ntiles = h.nstrips * h.nsegs;
+    resort_points(h, D->W, h.src_begin, h.src_j, h.src_i, h.src_val, h.src_off, h.src_len);
+    resort_points(h, D->W, h.ovr_begin, h.ovr_j, h.ovr_i, h.ovr_val, h.ovr_off, h.ovr_len);
+    char* A = D->arena;
+    auto up = [&](size_t off, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return AF_OK;
+        AF_HIP(hipMemcpyAsync(A + off, src, bytes, hipMemcpyHostToDevice, st));
+        return AF_OK;
+    };
+    int rc = AF_OK;
+    if ((rc = up(h.o_src_begin, h.src_begin.data(), h.src_begin.size() * 4))) return rc;
+    if ((rc = up(h.o_src_j, h.src_j.data(), h.src_j.size() * 4))) return rc;
+    if ((rc = up(h.o_src_i, h.src_i.data(), h.src_i.size() * 4))) return rc;
+    if ((rc = up(h.o_src_val, h.src_val.data(), h.src_val.size() * 8))) return rc;
+    if ((rc = up(h.o_src_off, h.src_off.data(), h.src_off.size() * 4))) return rc;
+    if ((rc = up(h.o_src_len, h.src_len.data(), h.src_len.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_begin, h.ovr_begin.data(), h.ovr_begin.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_j, h.ovr_j.data(), h.ovr_j.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_i, h.ovr_i.data(), h.ovr_i.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_val, h.ovr_val.data(), h.ovr_val.size() * 8))) return rc;
+    if ((rc = up(h.o_ovr_off, h.ovr_off.data(), h.ovr_off.size() * 4))) return rc;
+    if ((rc = up(h.o_ovr_len, h.ovr_len.data(), h.ovr_len.size() * 4))) return rc;
+    h.dev.tile_rows = R;
+    h.dev.ntiles = h.ntiles;
+    AF_HIP(hipMemcpyAsync(D->d_scns + s, &h.dev, sizeof(ScnDev), hipMemcpyHostToDevice, st));
+    AF_HIP(hipStreamSynchronize(st));  // the host vectors above are sources of the copies
+    return AF_OK;
+}
+
+// A pool that has drained below kMinTiles tiles runs its scenarios nearly
+// alone: those admitted under load, with taller tiles, move to the height
+// they would take alone (as long as the tile table has room).
+int retile_on_drain(af_domain* D, hipStream_t st) {
+    int used = 0;
+    for (const HostScn& h : D->scn) used += h.active ? h.ntiles : 0;
+    if (used >= kMinTiles) return AF_OK;
+    for (int s = 0; s < D->max_scn; s++) {
+        HostScn& h = D->scn[s];
+        if (!h.active || h.R <= h.R_alone) continue;
+        const int grow = h.cap_tiles - h.ntiles;
+        if (used + grow > D->max_tiles) continue;
+        int rc = retile(D, s, h.R_alone, st);
+        if (rc) return rc;
+        used += grow;
+    }
+    return AF_OK;
+}
+
 int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn, int max_tiles,
               void* arena, size_t arena_bytes) {
     int rc = configure(D, precision, tuning);
@@ -919,7 +1019,12 @@ int af_pool_release(af_domain* D, int32_t slot, af_stream_t stream) {
     HostScn& h = D->scn[slot];
     heap_free(D, h.base, h.bytes);
     h = HostScn{};
-    return rebuild_tiles(D, to_stream(stream));
+    hipStream_t st = to_stream(stream);
+    if (D->pool) {
+        int rc = retile_on_drain(D, st);
+        if (rc) return rc;
+    }
+    return rebuild_tiles(D, st);
 }
 
 int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finished,

@@ -132,3 +132,43 @@ def test_default_tile_height_follows_pool_load(gpu):
     assert shared.stats()["tiles"] == 2048             # 8 strips x 2048/8 segments
     shared.admit(small)
     assert shared.stats()["tiles"] == 2048 + 4         # 1 strip x 32/8 segments
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_pool_retiles_when_it_drains(gpu, prec):
+    """A scenario admitted beside a domain that fills the launch takes 8-row
+    tiles.  When that domain is released the pool holds fewer than 2048 tiles,
+    and the scenario moves, mid-run and with a mass report pending, to the
+    4-row tiles it would take alone.  Its depth grid and step count equal its
+    alone run bitwise; its reports have the same times, steps and wet areas,
+    and volumes equal to rounding (the partial sums are grouped by tile)."""
+    import dataclasses
+    from adaptflood_amd.domain import Batch
+    from adaptflood_amd.scenario import Scenario
+    small = dataclasses.replace(generators.small_case(128, 32, seed=141, sim_time=300.0),
+                                mass_int=1.0)
+    dem = np.tile(np.linspace(2.0, 0.0, 2048), (2048, 1))
+    big = Scenario(dem=dem, dx=5.0, sim_time=1e6)
+    pool = Pool(budget_bytes=2 << 30, precision=prec, max_scenarios=2, max_tiles=8192)
+    sb = pool.admit(big)
+    ss = pool.admit(small)
+    assert pool.stats()["tiles"] == 2048 + 4            # the small one at 8 rows
+    pool.advance(13, graph=False)
+    assert pool.info(ss)["t"] < small.sim_time
+    pool.release(sb)
+    assert pool.stats()["tiles"] == 8                   # now at 4 rows
+    for _ in range(10000):
+        pool.advance(6, graph=True)
+        fin, _ = pool.poll()
+        if ss in fin:
+            break
+    assert ss in fin
+    d = Domain(small, precision=prec)
+    d.run(check_every=6, graph=False)
+    print(f"{prec}: steps {pool.info(ss)['steps']}", flush=True)
+    assert pool.info(ss)["steps"] == d.info()["steps"]
+    assert np.array_equal(pool.read(ss, "H"), d.read("H"))
+    rp, rd = Batch.reports(pool, ss), d.reports()
+    assert rp.shape == rd.shape and len(rp) > 13
+    assert np.array_equal(rp[:, [0, 1, 3]], rd[:, [0, 1, 3]])
+    np.testing.assert_allclose(rp[:, 2], rd[:, 2], rtol=1e-12, atol=0)
