@@ -100,10 +100,16 @@ constexpr int kMinTiles = 2048;   // wavefronts a lone scenario should offer
 // Lowest default tile height: a scenario that runs by itself (a lone domain,
 // or one admitted to a pool holding fewer than kMinTiles tiles) goes down to
 // kMinRowsAlone rows; one that shares the launch with enough other tiles
-// stops at kMinRowsShared, where the re-read row above and below each segment
-// costs a quarter more reads instead of half.
-constexpr int kMinRowsAlone = 4;
+// stops at kMinRowsShared, where the rows re-read above and below each
+// segment cost a quarter more reads instead of half or more.
+constexpr int kMinRowsAlone = 2;
 constexpr int kMinRowsShared = 8;
+// Tiles shorter than 8 rows are taken only while the scenario stays under
+// kMinTiles tiles, and shorter than 4 only while it stays at or under
+// kMaxTilesTiny: a lone domain's step is bound by the instructions each SIMD
+// issues, and past that many tiles the per-tile work (north faces, point
+// lists, the first rows' loads) outweighs the shorter row march.
+constexpr int kMaxTilesTiny = 512;
 
 int elem_size(int precision) { return precision == AF_FP64 ? 8 : 4; }
 
@@ -174,13 +180,20 @@ void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
     h.nstrips = (d->nx + D->W - 1) / D->W;
     // Tile rows: the tuning's value, else the default height, halved while
     // the scenario alone would give fewer than kMinTiles wavefronts, down to
-    // D->min_rows.  A small domain run by itself is latency-bound: each
+    // D->min_rows (below 8 and 4 rows only while the tiles stay few; see
+    // kMaxTilesTiny).  A small domain run by itself is latency-bound: each
     // wavefront marches its rows one after another, so a step takes as long
     // as the tallest tile, and shorter tiles split the rows over more
     // wavefronts.
+    auto tiles = [&](int r) { return (long long)h.nstrips * ((d->ny + r - 1) / r); };
     auto halve = [&](int floor_rows) {
         int r = D->R;
-        while (r > floor_rows && (long long)h.nstrips * ((d->ny + r - 1) / r) < kMinTiles) r /= 2;
+        while (r > floor_rows && tiles(r) < kMinTiles) {
+            const int half = r / 2;
+            if (half < 8 && tiles(half) >= kMinTiles) break;
+            if (half < 4 && tiles(half) > kMaxTilesTiny) break;
+            r = half;
+        }
         return r;
     };
     int R = D->R_fixed, Ra = D->R_fixed;

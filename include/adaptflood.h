@@ -181,9 +181,12 @@ typedef struct af_domain af_domain; /* opaque, host side */
 /* Tile shape: each wavefront owns a strip of 64*V columns (V = 4 at FP32,
  * 2 at FP64) and marches down `tile_rows` rows.  0 = library default: 64 rows
  * at FP32 and 32 at FP64, halved per scenario while the scenario would offer
- * fewer than 2048 wavefronts, down to 4 rows for a scenario that runs alone
+ * fewer than 2048 wavefronts, down to 2 rows for a scenario that runs alone
  * (a domain, a batch of one, or an admission to a pool holding fewer than
- * 2048 tiles) and to 8 rows otherwise.  No field changes any result: the
+ * 2048 tiles) and to 8 rows otherwise; tiles under 8 rows only while the
+ * scenario stays under 2048 tiles, under 4 rows only while at most 512.  A
+ * pool that drains below 2048 tiles moves its scenarios to the height they
+ * would take alone.  No field changes any result: the
  * tile shape, variant and order only change how the work is split. */
 typedef struct af_tuning {
     int32_t tile_rows;

@@ -152,7 +152,7 @@ def test_tiling_invariance(gpu, prec):
     scn = _small(31, 2100, 141, sim_time=60.0)
     outs = []
     for rows, wpb, var, order in ((4, 1, 1, 0), (32, 4, 0, 0), (128, 4, 2, 0), (61, 4, 3, 1),
-                                  (33, 1, 2, 1)):
+                                  (33, 1, 2, 1), (2, 4, 2, 0), (1, 1, 1, 0)):
         d = Domain(scn, precision=prec,
                    tuning=make_tuning(tile_rows=rows, waves_per_block=wpb, variant=var,
                                       tile_order=order))

@@ -116,17 +116,17 @@ def test_request_over_budget_is_rejected(gpu):
 
 def test_default_tile_height_follows_pool_load(gpu):
     """Default tiling in a pool: a small scenario admitted to an empty pool
-    takes the shortest tiles (4 rows: it runs alone and is latency-bound);
+    takes the shortest tiles (2 rows: it runs alone and is latency-bound);
     admitted beside a domain that already offers 2048 tiles, it stops at 8
-    rows.  The footprint query reports the larger, 4-row tile count."""
+    rows.  The footprint query reports the larger, 2-row tile count."""
     from adaptflood_amd.scenario import Scenario
     small = generators.small_case(128, 32, seed=140, sim_time=30.0)
     dem = np.tile(np.linspace(2.0, 0.0, 2048), (2048, 1))
     big = Scenario(dem=dem, dx=5.0, sim_time=10.0)
     alone = Pool(budget_bytes=64 << 20, precision="fp32", max_scenarios=2, max_tiles=8192)
-    assert alone.footprint(small)[1] == 8              # 1 strip x 32/4 segments
+    assert alone.footprint(small)[1] == 16             # 1 strip x 32/2 segments
     alone.admit(small)
-    assert alone.stats()["tiles"] == 8
+    assert alone.stats()["tiles"] == 16
     shared = Pool(budget_bytes=1 << 30, precision="fp32", max_scenarios=2, max_tiles=8192)
     shared.admit(big)
     assert shared.stats()["tiles"] == 2048             # 8 strips x 2048/8 segments
@@ -139,7 +139,7 @@ def test_pool_retiles_when_it_drains(gpu, prec):
     """A scenario admitted beside a domain that fills the launch takes 8-row
     tiles.  When that domain is released the pool holds fewer than 2048 tiles,
     and the scenario moves, mid-run and with a mass report pending, to the
-    4-row tiles it would take alone.  Its depth grid and step count equal its
+    2-row tiles it would take alone.  Its depth grid and step count equal its
     alone run bitwise; its reports have the same times, steps and wet areas,
     and volumes equal to rounding (the partial sums are grouped by tile)."""
     import dataclasses
@@ -156,7 +156,7 @@ def test_pool_retiles_when_it_drains(gpu, prec):
     pool.advance(13, graph=False)
     assert pool.info(ss)["t"] < small.sim_time
     pool.release(sb)
-    assert pool.stats()["tiles"] == 8                   # now at 4 rows
+    assert pool.stats()["tiles"] == 16                  # now at 2 rows
     for _ in range(10000):
         pool.advance(6, graph=True)
         fin, _ = pool.poll()
