@@ -3,7 +3,7 @@
 // written in the classic 64-bit-offset NetCDF format.
 //
 // The reference creates a NetCDF-4 file through libnetcdf (HDF5 storage).
-// Neither library is part of this build, so the same dimensions, variables,
+// Neither library is linked here, so the same dimensions, variables,
 // attributes and records go into the classic format: big-endian, a header
 // followed by one record per save, readable by every NetCDF library.
 //
