@@ -369,3 +369,39 @@ def test_tile_with_more_points_than_the_batch(gpu, prec, rows):
         np.testing.assert_array_equal(d.read("Qx"), ref["Qx"])
     else:
         assert err <= FP32_TOL
+
+
+def test_many_scenarios_poll_and_run_to_their_own_ends(gpu):
+    """48 ragged scenarios, more than one block of the control-block gather:
+    a batch run stops when every scenario has reached its own stop time, each
+    equal (steps, H) to its solo run; a pool reports each slot finished in the
+    first poll after its clock reaches its stop time, and never before."""
+    from adaptflood_amd.serve import Pool
+    scns = [_small(300 + k, 20 + 7 * (k % 5), 12 + 5 * (k % 4), sim_time=30.0 + 4.0 * k)
+            for k in range(48)]
+    b = Batch(scns, precision="fp32", tuning=make_tuning(graph_steps=12))
+    b.run(check_every=12, graph=True)
+    for k, s in enumerate(scns):
+        inf = b.info(k)
+        assert inf["finished"] and inf["t"] >= s.sim_time, k
+    for k in range(0, 48, 8):
+        d = Domain(scns[k], precision="fp32")
+        d.run(check_every=12, graph=True)
+        assert b.info(k)["steps"] == d.info()["steps"], k
+        assert np.array_equal(b.read("H", k), d.read("H")), k
+    pool = Pool(budget_bytes=64 << 20, precision="fp32", max_scenarios=64, max_tiles=1 << 14)
+    slots = [pool.admit(s) for s in scns]
+    assert None not in slots
+    reported = {}
+    for it in range(4000):
+        pool.advance(6, graph=False)
+        fin, act = pool.poll()
+        assert act == 48
+        for sl in slots:
+            done = pool.info(sl)["t"] >= scns[slots.index(sl)].sim_time
+            assert (sl in fin) == done, (it, sl)
+            if done and sl not in reported:
+                reported[sl] = it
+        if len(reported) == 48:
+            break
+    assert len(reported) == 48
