@@ -12,7 +12,9 @@ oracle is timed on a bounded sample of the roofline workload ("cpu_baseline").
 Other workloads (--workload): "ensemble" (512 ragged scenarios in one launch
 per step against sequential single-scenario runs), "served" (the served
 ensemble with its sequential baselines), "accuracy" (FP32/FP64 errors against
-FP64 references) and "divergence" (lockstep drift at equal step counts).
+FP64 references), "divergence" (lockstep drift at equal step counts), "shard"
+(one GPU's row slab), "driver" (the rebuilt reference program end to end) and
+"lone" (step time of small and mid-size domains run alone).
 
 Algorithmic bytes per cell-update: read z, H, Qx_old, Qy_old and write H, Qx,
 Qy = 7 values = 28 B at FP32 (56 B at FP64).
@@ -544,6 +546,59 @@ def bench_driver(args):
         "async_only_files": len(extra)}), flush=True)
 
 
+def bench_lone(args):
+    """Step time of lone small and mid-size domains with the default tiling
+    (graph-replayed): the latency-bound regime of a single scenario, a served
+    request running by itself, or the sequential baselines.  Domains: wet flat
+    grids and members of the served ensemble generator (seed 5000+k)."""
+    import ctypes as C
+    from adaptflood_amd import generators
+    from adaptflood_amd._lib import call
+    from adaptflood_amd.scenario import Scenario
+
+    def flat(nx, ny):
+        dem = np.tile(np.linspace(2.0, 0.0, nx), (ny, 1))
+        return Scenario(dem=dem, dx=5.0, sim_time=1e9, h0=np.full((ny, nx), 0.5),
+                        name=f"flat {nx}x{ny}, wet")
+    cases = [flat(128, 32), flat(2048, 512)]
+    for k in (0, 7, 33):
+        m = generators.ensemble_member(k, seed_base=5000, sim_time=1e9)
+        m.name = f"ensemble member {k}"
+        cases.append(m)
+    import torch
+    from adaptflood_amd.domain import Domain
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    rows = []
+    for prec in (("fp32",) if args.no_fp64 else ("fp32", "fp64")):
+        W = 256 if prec == "fp32" else 128
+        for scn in cases:
+            d = Domain(scn, precision=prec)
+            tiles = C.c_int32()
+            call("af_pool_stats", d.handle, None, C.byref(tiles), None, None)
+            strips = (scn.nx + W - 1) // W
+            R = next(r for r in (64, 32, 16, 8, 4, 2, 1)
+                     if strips * ((scn.ny + r - 1) // r) == tiles.value)
+            d.advance(72, graph=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d.advance(args.lone_steps, graph=True)
+            torch.cuda.synchronize()
+            us = (time.perf_counter() - t0) / args.lone_steps * 1e6
+            d.close()
+            row = {"domain": scn.name, "nx": scn.nx, "ny": scn.ny, "points": len(scn.points),
+                   "precision": prec, "tiles": tiles.value, "tile_rows": R,
+                   "us_per_step": round(us, 2)}
+            rows.append(row)
+            print(f"[bench] lone {row}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "lone-domain step time (default tiling)", "unit": "us",
+                      "higher_is_better": False, "steps": args.lone_steps,
+                      "config": {"workload": "wet flat grids and served-ensemble members, "
+                                             "each alone, graph-replayed steps"},
+                      "domains": rows}), flush=True)
+
+
 def bench_shard(args):
     """One GPU's share of the 16384^2 roofline case split into `--parts` row
     slabs: slab 0 (16384 x 2048 at 8 parts) with its own point sources.  The
@@ -601,7 +656,7 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
                     choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard",
-                             "driver"),
+                             "driver", "lone"),
                     default="roofline")
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
@@ -625,6 +680,7 @@ def main():
     ap.add_argument("--plain-stride", type=int, default=16,
                     help="sequential plain baseline: every n-th scenario")
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--lone-steps", type=int, default=720, help="lone: timed steps per domain")
     ap.add_argument("--rate", type=float, default=20.0, help="served arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served HBM budget (GiB)")
     ap.add_argument("--chunk", type=int, default=36, help="served steps per pool advance")
@@ -648,6 +704,8 @@ def main():
         return bench_shard(args)
     if args.workload == "driver":
         return bench_driver(args)
+    if args.workload == "lone":
+        return bench_lone(args)
     served_reqs = None
     if not args.no_served:
         # generated in worker processes before this process touches the GPU
