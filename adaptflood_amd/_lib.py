@@ -14,6 +14,8 @@ AF_FP32 = 0
 AF_FP64 = 1
 AF_BC_CLOSED, AF_BC_FREE, AF_BC_HFIX, AF_BC_HVAR, AF_BC_QFIX, AF_BC_QVAR = range(6)
 AF_FLOODPLAINQ_WORKSPACE_BYTES = 256
+AF_SCN_OK, AF_SCN_NONFINITE, AF_SCN_STALLED = 0, 1, 2
+AF_E_INVALID, AF_E_CAPACITY, AF_E_STATE = -1, -4, -5
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -43,6 +45,7 @@ class RunInfo(C.Structure):
     _fields_ = [
         ("t", C.c_double), ("steps_done", C.c_int64), ("last_dt", C.c_double),
         ("n_reports", C.c_int32), ("finished", C.c_int32), ("hmax", C.c_double),
+        ("status", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -119,6 +122,7 @@ SIGNATURES = [
     ("af_pool_admit", _I, [_V, C.POINTER(ScenarioDesc), _V, _ip]),
     ("af_pool_release", _I, [_V, _I32, _V]),
     ("af_pool_poll", _I, [_V, _ip, _I32, _ip, _ip, _V]),
+    ("af_pool_progress", _I, [_V, _ip, _dp, C.POINTER(_I64), _ip, _ip, _I32, _ip, _V]),
     ("af_pool_stats", _I, [_V, _ip, _ip, C.POINTER(C.c_size_t), C.POINTER(_I64)]),
     ("af_par_defaults", None, [C.POINTER(Par)]),
     ("af_read_par", _I, [C.c_char_p, C.POINTER(Par)]),

@@ -535,7 +535,10 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
                  AsyncWriter* aw) {
     const size_t n = (size_t)d.nx * d.ny, nq = (size_t)(d.nx + 1) * (d.ny + 1);
     af_tuning tun{};
-    const double nrep = d.mass_int > 0 ? std::ceil(d.sim_time / d.mass_int) + 4 : 4;
+    // one record per report; massint <= 0 reports after every step
+    // (main.cpp:291-303), a count not known in advance: then the largest
+    // buffer (1M records, 32 MiB), and reports past it are counted, not shown
+    const double nrep = d.mass_int > 0 ? std::ceil(d.sim_time / d.mass_int) + 4 : 1048576.0;
     tun.report_cap = (int32_t)std::min(nrep, 1048576.0);
     // dt of the first 4M steps, for the Step lines
     const bool step_lines = !out.quiet;
@@ -578,14 +581,18 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     std::vector<double> dts;
     long long printed = 0;  // steps whose lines are out
     double t_run = 0.0;     // start time of step `printed`
+    bool dropped_line = false;
     auto show_progress = [&]() {
         af_run_info now;
         af_ok(af_domain_info(D, &now, nullptr), "info");
+        // records held on the device: the first report_cap of them
+        const int32_t held = std::min(now.n_reports, tun.report_cap);
         int32_t nrep = shown;
-        if (now.n_reports > shown) {
-            rep.assign(4 * (size_t)now.n_reports, 0.0);
-            af_ok(af_domain_read_reports(D, rep.data(), now.n_reports, &nrep, nullptr),
-                  "reports");
+        if (held > shown) {
+            rep.assign(4 * (size_t)held, 0.0);
+            int32_t total = 0;
+            af_ok(af_domain_read_reports(D, rep.data(), held, &total, nullptr), "reports");
+            nrep = held;
         }
         int64_t logged = 0;
         if (step_lines) {
@@ -608,6 +615,10 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
         for (; shown < nrep; shown++)  // past the logged steps
             report_line(rep[4 * shown], rep[4 * shown + 1], rep[4 * shown + 2],
                         rep[4 * shown + 3]);
+        if (now.n_reports > held && !dropped_line) {
+            printf("(mass reports after the first %d are not recorded)\n", held);
+            dropped_line = true;
+        }
         fflush(stdout);
     };
 
@@ -617,6 +628,9 @@ Result run_fused(const af_scenario_desc& d, const af_par& par, const Output& out
     while (info.t < d.sim_time) {
         double stop = d.sim_time;
         if (par.save_int > 0 && next_save < stop) stop = next_save;
+        // saveint <= 0 saves after every step, as main.cpp:326 does when
+        // next_save never passes t
+        if (par.save_int <= 0) stop = info.t;
         if (gauges && next_stage < stop) stop = next_stage;  // interval 0: every step
         if (info.t >= stop) {
             // one step jumped past two output times: the reference writes

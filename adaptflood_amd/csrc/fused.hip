@@ -120,7 +120,17 @@ int validate(const af_scenario_desc* d) {
     AF_REQUIRE((long long)d->nx * d->ny < (1LL << 31), AF_E_INVALID,
                "grid %d x %d: more than 2^31 cells", d->nx, d->ny);
     AF_REQUIRE(d->dem, AF_E_INVALID, "null DEM");
-    AF_REQUIRE(d->dx > 0.0, AF_E_INVALID, "cell size must be positive");
+    AF_REQUIRE(d->dx > 0.0 && std::isfinite(d->dx), AF_E_INVALID,
+               "cell size must be positive and finite");
+    // a NaN or infinite parameter would reach every cell on the first step;
+    // it is refused here instead (the reference runs it and returns NaN)
+    const struct { const char* name; double v; } params[] = {
+        {"fpn", d->fpn}, {"depth_thresh", d->depth_thresh}, {"max_hflow", d->max_hflow},
+        {"init_tstep", d->init_tstep}, {"cfl", d->cfl}, {"g", d->g},
+        {"sim_time", d->sim_time}, {"mass_int", d->mass_int}};
+    for (const auto& p : params)
+        AF_REQUIRE(std::isfinite(p.v), AF_E_INVALID, "%s is not finite (%g)", p.name, p.v);
+    AF_REQUIRE(d->nodata == d->nodata, AF_E_INVALID, "NODATA value is NaN");
     AF_REQUIRE(d->theta == 1.0, AF_E_INVALID,
                "the fused step implements theta = 1 (main.cpp:124); got %g", d->theta);
     AF_REQUIRE(d->n_points >= 0, AF_E_INVALID, "negative point count");
@@ -141,10 +151,21 @@ int validate(const af_scenario_desc* d) {
                    "point %d: series [%d, %d) outside %d entries", p, off, off + len,
                    d->n_series);
         if (len > 0) AF_REQUIRE(d->series_t && d->series_v, AF_E_INVALID, "null series arrays");
+        // a FREE point's value is not used (kernels.cpp:266-268)
+        AF_REQUIRE(ty == AF_BC_FREE || ty == AF_BC_CLOSED || std::isfinite(d->pt_value[p]),
+                   AF_E_INVALID, "point %d: value %g is not finite", p, d->pt_value[p]);
+        for (int e = off; e < off + len; e++)
+            AF_REQUIRE(std::isfinite(d->series_t[e]) && std::isfinite(d->series_v[e]),
+                       AF_E_INVALID, "point %d: series entry %d is not finite (%g at t = %g)", p,
+                       e - off, d->series_v[e], d->series_t[e]);
     }
     if (d->rain_enabled) {
         AF_REQUIRE(d->n_rain > 0 && d->rain_t && d->rain_r, AF_E_INVALID,
                    "rain enabled without a series");
+        for (int e = 0; e < d->n_rain; e++)
+            AF_REQUIRE(std::isfinite(d->rain_t[e]) && std::isfinite(d->rain_r[e]), AF_E_INVALID,
+                       "rain entry %d is not finite (%g at t = %g)", e, d->rain_r[e],
+                       d->rain_t[e]);
     }
     return AF_OK;
 }
@@ -386,9 +407,13 @@ size_t heap_free_bytes(const af_domain* D) {
     return n;
 }
 
+// The DEM and the initial depths in the device layout and precision.  Returns
+// AF_E_INVALID for a non-finite DEM value (other than a non-finite NODATA
+// value itself) or initial depth: such a cell would turn its neighbours'
+// fluxes to NaN on the first step.
 template <typename T>
-void convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>& z,
-                    std::vector<T>& H, double& zmin, double& hmax) {
+int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>& z,
+                   std::vector<T>& H, double& zmin, double& hmax) {
     const long long P = h.P;
     const bool f64 = sizeof(T) == 8;
     zmin = 0.0;
@@ -415,11 +440,17 @@ void convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>&
             if (f64) zt = (T)v;
             else zt = nod ? (T)kNodataF32 : (T)(v - zmin);
             z[(size_t)j * P + i] = zt;
+            AF_REQUIRE(nod || std::isfinite(v), AF_E_INVALID,
+                       "DEM value at column %d, row %d is not finite (%g)", i, j, v);
             T ht = d->h0 ? (T)d->h0[c] : T(0);
+            AF_REQUIRE(std::isfinite((double)ht), AF_E_INVALID,
+                       "initial depth at column %d, row %d is not finite (%g)", i, j,
+                       d->h0[c]);
             H[(size_t)j * P + i] = ht;
             if (!nod && (double)ht > hmax) hmax = (double)ht;  // kernels.cpp:32
         }
     }
+    return AF_OK;
 }
 
 int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t st,
@@ -499,13 +530,15 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     double hmax = -DBL_MAX;
     if (D->precision == AF_FP64) {
         std::vector<double> z, H;
-        convert_inputs<double>(d, h, z, H, h.zmin, hmax);
+        int crc = convert_inputs<double>(d, h, z, H, h.zmin, hmax);
+        if (crc) return crc;
         AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipStreamSynchronize(st));
     } else {
         std::vector<float> z, H;
-        convert_inputs<float>(d, h, z, H, h.zmin, hmax);
+        int crc = convert_inputs<float>(d, h, z, H, h.zmin, hmax);
+        if (crc) return crc;
         AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipStreamSynchronize(st));
@@ -693,6 +726,18 @@ int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
     AF_HIP(hipMemcpyAsync(out.data(), D->d_ctlbuf, sizeof(Ctl) * n, hipMemcpyDeviceToHost, st));
     AF_HIP(hipStreamSynchronize(st));
     return AF_OK;
+}
+
+// A slot is finished when its clock reached sim_time (main.cpp:227) or a
+// non-zero status stopped it.
+bool slot_finished(const af_domain* D, int s, const Ctl& c, int par) {
+    return !(c.t[par] < D->scn[s].dev.sim_time) || c.status != 0;
+}
+
+const char* status_text(int status) {
+    if (status & kStatusNonfinite) return "a step produced a NaN or infinite depth";
+    if (status & kStatusStalled) return "the clock stalled (t + dt == t)";
+    return "unknown status";
 }
 
 // Rebuilds and uploads the tile table from the active scenarios; a captured
@@ -1051,13 +1096,34 @@ int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finish
     int nf = 0;
     for (size_t a = 0; a < D->active.size(); a++) {
         const int s = D->active[a];
-        if (!(c[a].t[par] < D->scn[s].dev.sim_time)) {
+        if (slot_finished(D, s, c[a], par)) {
             if (finished && nf < cap) finished[nf] = s;
             nf++;
         }
     }
     *n_finished = nf;
     if (n_active) *n_active = (int)D->active.size();
+    return AF_OK;
+}
+
+int af_pool_progress(af_domain* D, int32_t* slots, double* t, int64_t* steps, int32_t* status,
+                     int32_t* finished, int32_t cap, int32_t* n, af_stream_t stream) {
+    AF_REQUIRE(D && n && cap >= 0, AF_E_INVALID, "progress: bad arguments");
+    hipStream_t st = to_stream(stream);
+    const int par = (int)(D->k & 1);
+    std::vector<Ctl> c;
+    int rc = gather_ctl(D, c, st);
+    if (rc) return rc;
+    const int na = (int)D->active.size();
+    for (int a = 0; a < na && a < cap; a++) {
+        const int s = D->active[a];
+        if (slots) slots[a] = s;
+        if (t) t[a] = c[a].t[par];
+        if (steps) steps[a] = c[a].n_done;
+        if (status) status[a] = c[a].status;
+        if (finished) finished[a] = slot_finished(D, s, c[a], par) ? 1 : 0;
+    }
+    *n = na;
     return AF_OK;
 }
 
@@ -1113,7 +1179,7 @@ int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
     const int par = (int)(D->k & 1);
     int done = 1;
     for (size_t a = 0; a < D->active.size(); a++)
-        if (c[a].t[par] < D->scn[D->active[a]].dev.sim_time) done = 0;
+        if (!slot_finished(D, D->active[a], c[a], par)) done = 0;
     *all_done = done;
     return AF_OK;
 }
@@ -1131,11 +1197,21 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
         long long steps = 0;
         double tsum = 0.0;
         for (size_t a = 0; a < D->active.size(); a++) {
-            if (c[a].t[par] < D->scn[D->active[a]].dev.sim_time) done = 0;
+            if (!slot_finished(D, D->active[a], c[a], par)) done = 0;
             steps += c[a].n_done;
             tsum += c[a].t[par];
         }
-        if (done) break;
+        if (done) {
+            // every scenario ran to its sim_time or stopped; name the first
+            // that stopped
+            for (size_t a = 0; a < D->active.size(); a++) {
+                if (c[a].status == 0) continue;
+                set_error("scenario %d stopped after step %lld at t = %.17g: %s", D->active[a],
+                          (long long)c[a].n_done, c[a].t[par], status_text(c[a].status));
+                return AF_E_STATE;
+            }
+            break;
+        }
         // a clock that no longer advances (dt == 0 after an overflow) would never end
         if (prev_done >= 0 && steps > prev_done && !(tsum > D->last_tsum)) {
             set_error("simulation clock stalled at t = %g", tsum);
@@ -1176,8 +1252,10 @@ int af_batch_info(af_domain* D, int s, af_run_info* info, af_stream_t stream) {
     info->steps_done = c.n_done;
     info->last_dt = c.last_dt;
     info->n_reports = c.n_reports;
-    info->finished = !(c.t[tpar] < D->scn[s].dev.sim_time);
+    info->finished = slot_finished(D, s, c, tpar) ? 1 : 0;
     info->hmax = fkey_inv_d(c.hmax[(int)(D->k % 3)]);
+    info->status = c.status;
+    info->reserved = 0;
     return AF_OK;
 }
 

@@ -40,8 +40,12 @@ struct Ctl {
     double pending_t, pending_dt;
     int n_reports;
     int par[2];                   // buffer holding the state entering step k: par[k & 1]
-    int pad_;
+    int status;                   // AF_SCN_* bits; non-zero stops the scenario for good
 };
+
+// Ctl.status bits (the values of AF_SCN_NONFINITE / AF_SCN_STALLED).
+constexpr int kStatusNonfinite = 1;   // a step produced a NaN or infinite depth
+constexpr int kStatusStalled = 2;     // t + dt == t: the clock cannot advance
 
 // Per-scenario constants and device pointers.
 struct ScnDev {
@@ -209,6 +213,7 @@ struct TileState {
     int sp, se, sk, sn, src_next_j;   // sources: first, end, next (from sp), batched; next row
     int op, oe, ok, on, ovr_next_j;   // overrides: likewise
     T hmax;
+    bool finite;                  // every depth this lane produced is finite
     double vol_s, vol_c;
     unsigned long long wet;
 };
@@ -349,6 +354,9 @@ struct Stepper {
         for (int v = 0; v < V; v++) {
             if (!EDGE || c0 + v < nx) {
                 if (c.z[v] != nod && hn[v] > st.hmax) st.hmax = hn[v];
+                // NaN never wins the maximum (as in kernels.cpp:32), so it is
+                // looked for separately
+                st.finite = st.finite && __builtin_isfinite(hn[v]);
                 if (x.report_due) {
                     const double hd = (double)hn[v];
                     kahan_add(st.vol_s, st.vol_c, hd * x.dA);
@@ -558,12 +566,19 @@ struct Stepper {
         const double t = ctl->t[cur];
         const double next_mass = ctl->next_mass[cur];
         const double hmax_in = fkey_inv_d(ctl->hmax[hs_cur]);
-        const bool skip = !(t < S.sim_time);  // main.cpp:227
+        const int status = ctl->status;
         const double dt = cfl_dt(hmax_in, S.thresh, S.cfl, S.dx, S.g, S.init_tstep);
+        // a step that cannot move the clock (dt lost against t) would repeat
+        // forever at the same t: every wave sees the same t and dt, so all of
+        // them skip it, and the first tile's wave marks the scenario stalled
+        const bool running = (t < S.sim_time) && status == 0;  // main.cpp:227
+        const bool stalled = running && !(t + dt > t);
+        const bool skip = !running || stalled;
         const bool report_due = !skip && (t + dt >= next_mass);
 
         if (td.first) bookkeeping(S, cur, nxt, hs_cur, hs_nxt, hs_rst, s_in, t, dt, next_mass,
                                   skip, report_due, lane);
+        if (td.first && stalled && lane == 0) atomicOr(&S.ctl->status, kStatusStalled);
         if (skip) return;
 
         TileCtx<T> x;
@@ -670,6 +685,7 @@ struct Stepper {
         st.ovr_next_j = INT_MAX;
         if (st.sp < st.se || st.op < st.oe) load_points(S, x, st);
         st.hmax = F64 ? T(-DBL_MAX) : T(-FLT_MAX);
+        st.finite = true;
         st.vol_s = 0.0;
         st.vol_c = 0.0;
         st.wet = 0;
@@ -696,6 +712,11 @@ struct Stepper {
                 row2<EDGE>(S, x, st, C, D, A, B, j + 2);
             }
         }
+
+        // a non-finite depth stops the scenario from the next step on (the
+        // bit is sticky; steps already under way finish or skip, and the
+        // scenario's results are not used)
+        if (__any(!st.finite) && lane == 0) atomicOr(&S.ctl->status, kStatusNonfinite);
 
         // wavefront max, one atomic per wavefront
         T hm = st.hmax;

@@ -98,7 +98,8 @@ class Batch:
         r = RunInfo()
         call("af_batch_info", self.handle, i, C.byref(r), self._s())
         return {"t": r.t, "steps": r.steps_done, "last_dt": r.last_dt,
-                "n_reports": r.n_reports, "finished": bool(r.finished), "hmax": r.hmax}
+                "n_reports": r.n_reports, "finished": bool(r.finished), "hmax": r.hmax,
+                "status": r.status}
 
     def set_sim_time(self, sim_time, i=0):
         """Moves the stop time of scenario i (runs stop at the first step with

@@ -14,10 +14,21 @@ import time
 import numpy as np
 
 from . import _lib
-from ._lib import call
+from ._lib import AF_E_CAPACITY, AF_E_INVALID, AF_SCN_NONFINITE, AF_SCN_STALLED, call
 from .domain import PRECISIONS, _stream, _torch, make_tuning
 
-AF_E_CAPACITY = -4
+
+class RequestError(ValueError):
+    """A scenario the pool refuses for what it is (a non-finite input, a bad
+    grid), as opposed to a pool that has no room for it right now."""
+
+
+def status_text(status):
+    if status & AF_SCN_NONFINITE:
+        return "a step produced a NaN or infinite depth"
+    if status & AF_SCN_STALLED:
+        return "the clock stalled (t + dt == t)"
+    return f"status {status}"
 
 
 class Pool:
@@ -65,12 +76,15 @@ class Pool:
         return int(b.value), int(t.value)
 
     def admit(self, scn):
-        """Slot index, or None when the pool has no room right now."""
+        """Slot index, or None when the pool has no room right now; raises
+        RequestError for a scenario the pool refuses (bad input)."""
         pk = scn.pack()
         slot = C.c_int32()
         rc = _lib.lib().af_pool_admit(self.handle, C.byref(pk.desc), self._s(), C.byref(slot))
         if rc == AF_E_CAPACITY:
             return None
+        if rc == AF_E_INVALID:
+            raise RequestError(_lib.lib().af_last_error().decode(errors="replace"))
         _lib.check("af_pool_admit", rc)
         self.packed[slot.value] = (scn, pk)
         return slot.value
@@ -86,12 +100,19 @@ class Pool:
         self._torch.cuda.synchronize()
 
     def poll(self):
-        """(finished slots, number of active slots)."""
+        """(finished slots, number of active slots, {active slot: t}), from
+        one gather of the active control blocks.  A slot is finished when it
+        reached its sim_time or a non-zero status stopped it."""
         cap = max(1, len(self.packed))
-        buf = (C.c_int32 * cap)()
-        nf, na = C.c_int32(), C.c_int32()
-        call("af_pool_poll", self.handle, buf, cap, C.byref(nf), C.byref(na), self._s())
-        return [buf[i] for i in range(min(nf.value, cap))], na.value
+        slots = (C.c_int32 * cap)()
+        t = (C.c_double * cap)()
+        fin = (C.c_int32 * cap)()
+        n = C.c_int32()
+        call("af_pool_progress", self.handle, slots, t, None, None, fin, cap, C.byref(n),
+             self._s())
+        m = min(n.value, cap)
+        clocks = {slots[a]: t[a] for a in range(m)}
+        return [slots[a] for a in range(m) if fin[a]], n.value, clocks
 
     def stats(self):
         a, t = C.c_int32(), C.c_int32()
@@ -104,7 +125,8 @@ class Pool:
     def info(self, slot):
         r = _lib.RunInfo()
         call("af_batch_info", self.handle, int(slot), C.byref(r), self._s())
-        return {"t": r.t, "steps": r.steps_done, "last_dt": r.last_dt, "hmax": r.hmax}
+        return {"t": r.t, "steps": r.steps_done, "last_dt": r.last_dt, "hmax": r.hmax,
+                "status": r.status}
 
     def read(self, slot, what="H"):
         which = {"H": 0, "Qx": 1, "Qy": 2, "Qxold": 3, "Qyold": 4}[what]
@@ -119,7 +141,7 @@ class Pool:
 
 class Request:
     __slots__ = ("rid", "scenario", "arrival", "admitted", "done", "steps", "cells", "result",
-                 "error")
+                 "error", "last_t", "stalls")
 
     def __init__(self, rid, scenario, arrival):
         self.rid, self.scenario, self.arrival = rid, scenario, arrival
@@ -128,28 +150,36 @@ class Request:
         self.cells = scenario.cells
         self.result = None
         self.error = None
+        self.last_t = None
+        self.stalls = 0
 
 
-def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, graph=True):
+def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, graph=True,
+          stall_polls=8):
     """Runs requests (a list of Request, sorted by arrival) through the pool.
 
-    The pool needs admit(scenario) -> slot or None, release(slot),
-    advance(steps, graph), poll() -> (finished slots, n_active),
-    info(slot) -> {"steps": ...}, read(slot, "H") and synchronize().
+    The pool needs admit(scenario) -> slot or None (raising RequestError for a
+    scenario it refuses), release(slot), advance(steps, graph),
+    poll() -> (finished slots, n_active, {active slot: t}),
+    info(slot) -> {"steps", "t", "status"}, read(slot, "H") and synchronize().
 
     clock="wall": arrivals are seconds on the wall clock from the start of the
     call (a request is admitted once it has arrived); clock="steps": arrivals
     are pool step indices (a deterministic schedule for tests).  Requests are
     admitted in arrival order.  One that an empty pool cannot take (larger
-    than its HBM budget, slots or tile table) is rejected with `error` set
-    rather than waited for.  Returns the completed requests, with admission
-    and completion times filled in, and a summary (which counts the rejected
-    ones)."""
+    than its HBM budget, slots or tile table) or that the pool refuses (a
+    non-finite input) is rejected with `error` set rather than waited for.
+    A request that stops on the device (a NaN or infinite depth, a stalled
+    clock), or whose clock does not move over `stall_polls` polls in a row,
+    is released with `error` set; the others keep running.  Returns the
+    completed requests, with admission and completion times filled in, and a
+    summary (which counts the rejected and failed ones)."""
     pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
     waiting = collections.deque()
     active = {}
     done = []
     rejected = []
+    failed = []
     t0 = time.perf_counter()
     step = 0
     cell_updates = 0
@@ -157,13 +187,25 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
     def now():
         return time.perf_counter() - t0 if clock == "wall" else float(step)
 
+    def fail(slot, r, why):
+        r.error = why
+        r.done = now()
+        pool.release(slot)
+        failed.append(r)
+
     while pending or waiting or active:
         n = now()
         while pending and pending[0].arrival <= n:
             waiting.append(pending.popleft())
         while waiting:
             r = waiting[0]
-            slot = pool.admit(r.scenario)
+            try:
+                slot = pool.admit(r.scenario)
+            except RequestError as e:
+                waiting.popleft()           # refused for what it is: it never will fit
+                r.error = f"refused: {e}"
+                rejected.append(r)
+                continue
             if slot is None:
                 if active:
                     break                   # room frees up as active requests finish
@@ -173,6 +215,7 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
                 continue
             waiting.popleft()
             r.admitted = now()
+            r.last_t, r.stalls = None, 0
             active[slot] = r
         if not active:
             if clock == "wall":
@@ -183,23 +226,40 @@ def serve(requests, pool, chunk_steps=36, clock="wall", keep_results=False, grap
             continue
         pool.advance(chunk_steps, graph=graph)
         step += chunk_steps
-        finished, _ = pool.poll()
+        finished, _, clocks = pool.poll()
         for slot in finished:
             r = active.pop(slot)
             inf = pool.info(slot)
             r.steps = inf["steps"]
+            status = inf.get("status", 0)
+            if status:
+                fail(slot, r, f"stopped after step {r.steps} at t = {inf.get('t', 0.0):.6g} s: "
+                              f"{status_text(status)}")
+                continue
             cell_updates += r.steps * r.cells
             if keep_results:
                 r.result = pool.read(slot, "H")
             r.done = now()
             pool.release(slot)
             done.append(r)
+        # a clock that no longer moves would hold its slot for ever
+        for slot, r in list(active.items()):
+            t = clocks.get(slot)
+            if t is None:
+                continue
+            r.stalls = r.stalls + 1 if (r.last_t is not None and not t > r.last_t) else 0
+            r.last_t = t
+            if r.stalls >= stall_polls:
+                active.pop(slot)
+                r.steps = pool.info(slot)["steps"]
+                fail(slot, r, f"clock stalled at t = {t:.6g} s over {r.stalls} polls")
     pool.synchronize()
     wall = time.perf_counter() - t0
     lat = np.array([r.done - r.arrival for r in done]) if done else np.zeros(1)
     summary = {
         "requests": len(done),
         "rejected": len(rejected),
+        "failed": len(failed),
         "wall_s": wall,
         "cell_updates": cell_updates,
         "gcups": cell_updates / wall / 1e9 if clock == "wall" else None,

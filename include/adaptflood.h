@@ -92,7 +92,9 @@ int af_BCs_GPU(double* Qx, double* Qy, const double* H, const double* DEM, int x
  * share a cell resolve in array order (the last one wins).  ncols, dx, dA and
  * Tstep are unused, as in the reference.  Every bc_idx must lie in
  * [0, xsz*ysz): like the reference, the call does not know the grid size.
- * Duplicates are found by a scan of the point list (O(num_bcs^2) work). */
+ * Up to 8192 points are grouped by cell with one sort in LDS (O(n log^2 n)
+ * work in one workgroup); longer lists fall back to a scan of the list for
+ * each point (O(num_bcs^2) work). */
 int af_ApplyPointBCs_GPU(double* H, const int32_t* bc_idx, const int32_t* bc_type,
                          const double* bc_value, int num_bcs, int ncols, double dx, double dA,
                          double Tstep, af_stream_t stream);
@@ -109,8 +111,9 @@ int af_Rainfall_GPU(double* H, const double* DEM, int xsz, int ysz, double rain_
 
 /* Replaces the inline AddFlowBCs kernel (src/main.cpp:256-260):
  * H[ps_idx[s]] += ps_flow[s], clamped at 0.  Sources that share a cell are
- * applied in array order.  Every ps_idx must lie in [0, xsz*ysz).  Duplicates
- * are found by a scan of the source list (O(num_flow_bcs^2) work). */
+ * applied in array order.  Every ps_idx must lie in [0, xsz*ysz).  Up to
+ * 8192 sources are grouped by cell with one sort in LDS; longer lists fall
+ * back to a scan of the list for each source (O(num_flow_bcs^2) work). */
 int af_AddFlowBCs_GPU(double* H, const int32_t* ps_idx, const double* ps_flow, int num_flow_bcs,
                       af_stream_t stream);
 
@@ -167,13 +170,26 @@ typedef struct af_scenario_desc {
     const double* h0;        /* nx*ny */
 } af_scenario_desc;
 
+/* Scenario status bits (af_run_info.status).  A scenario with a non-zero
+ * status has stopped for good: its tiles skip every later step, and it
+ * counts as finished.  The reference has no such stop: its H max skips NaN
+ * (src/kernels.cpp:30-33), so a NaN depth runs to the end unnoticed, and an
+ * infinite depth gives dt = 0 and a loop that never ends (main.cpp:227). */
+enum {
+    AF_SCN_OK = 0,
+    AF_SCN_NONFINITE = 1,    /* a step produced a NaN or infinite depth */
+    AF_SCN_STALLED = 2       /* t + dt == t: the clock could not advance */
+};
+
 typedef struct af_run_info {
     double t;                /* simulated time after the last executed step */
     int64_t steps_done;      /* steps executed since af_domain_create */
     double last_dt;          /* Tstep of the last executed step */
     int32_t n_reports;       /* mass reports recorded so far */
-    int32_t finished;        /* t >= sim_time */
+    int32_t finished;        /* t >= sim_time, or stopped (status != 0) */
     double hmax;             /* max H over valid cells of the current state */
+    int32_t status;          /* AF_SCN_* bits */
+    int32_t reserved;
 } af_run_info;
 
 typedef struct af_domain af_domain; /* opaque, host side */
@@ -291,9 +307,17 @@ int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, in
 int af_pool_admit(af_domain* pool, const af_scenario_desc* desc, af_stream_t stream,
                   int32_t* slot);
 int af_pool_release(af_domain* pool, int32_t slot, af_stream_t stream);
-/* Slots whose clock reached their sim_time (up to cap), and the active count. */
+/* Slots whose clock reached their sim_time or that stopped with a non-zero
+ * status (up to cap), and the active count. */
 int af_pool_poll(af_domain* pool, int32_t* finished, int32_t cap, int32_t* n_finished,
                  int32_t* n_active, af_stream_t stream);
+/* Every active slot's progress, read with one gather launch, one copy and one
+ * synchronisation: slot index, t, executed steps, status (AF_SCN_*) and
+ * whether it is finished (t >= sim_time or status != 0).  Entry a of each
+ * non-null array describes the a-th active slot, for a < min(cap, *n);
+ * *n is the number of active slots. */
+int af_pool_progress(af_domain* pool, int32_t* slots, double* t, int64_t* steps, int32_t* status,
+                     int32_t* finished, int32_t cap, int32_t* n, af_stream_t stream);
 int af_pool_stats(af_domain* pool, int32_t* active, int32_t* tiles, size_t* heap_free,
                   int64_t* step_index);
 

@@ -96,3 +96,34 @@ def test_create_rejects_small_arena_before_gpu_work():
     rc = _lib.lib().af_domain_create(C.byref(pk.desc), 0, None, C.c_void_p(addr), 256, None,
                                      C.byref(h))
     assert rc == -4 and b"arena too small" in _lib.lib().af_last_error()
+
+
+def test_non_finite_inputs_are_refused_before_gpu_work():
+    """A NaN or infinite parameter, point value, series entry or rain entry
+    would reach every cell on the first step (the reference runs it and
+    returns NaN); the fused path refuses it when the scenario is sized."""
+    import dataclasses
+    base = generators.small_case(40, 30, seed=1)
+    assert _bytes(base) > 0
+    for field in ("fpn", "cfl", "depth_thresh", "init_tstep", "sim_time", "mass_int", "dx"):
+        for v in (np.nan, np.inf):
+            with pytest.raises(AFError, match="finite"):
+                _bytes(dataclasses.replace(base, **{field: v}))
+    with pytest.raises(AFError, match="NaN"):
+        _bytes(dataclasses.replace(base, nodata=np.nan))
+    qfix = next(k for k, p in enumerate(base.points) if p.type == _lib.AF_BC_QFIX)
+    qvar = next(k for k, p in enumerate(base.points) if p.type == _lib.AF_BC_QVAR)
+    free = next(k for k, p in enumerate(base.points) if p.type == _lib.AF_BC_FREE)
+    pts = [dataclasses.replace(p) for p in base.points]
+    pts[qfix].value = np.inf
+    with pytest.raises(AFError, match="point .*not finite"):
+        _bytes(dataclasses.replace(base, points=pts))
+    pts = [dataclasses.replace(p) for p in base.points]
+    pts[qvar].values = np.array([0.0, np.nan, 0.0])
+    with pytest.raises(AFError, match="series entry 1 is not finite"):
+        _bytes(dataclasses.replace(base, points=pts))
+    pts = [dataclasses.replace(p) for p in base.points]
+    pts[free].value = np.nan       # a FREE point's value is never used
+    assert _bytes(dataclasses.replace(base, points=pts)) > 0
+    with pytest.raises(AFError, match="rain entry 1"):
+        _bytes(dataclasses.replace(base, rain_rates=np.array([1e-5, np.inf, 0.0])))

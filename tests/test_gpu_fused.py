@@ -395,7 +395,7 @@ def test_many_scenarios_poll_and_run_to_their_own_ends(gpu):
     reported = {}
     for it in range(4000):
         pool.advance(6, graph=False)
-        fin, act = pool.poll()
+        fin, act, _ = pool.poll()
         assert act == 48
         for sl in slots:
             done = pool.info(sl)["t"] >= scns[slots.index(sl)].sim_time

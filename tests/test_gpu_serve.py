@@ -48,7 +48,7 @@ def test_pool_capacity_and_release(gpu):
     s2 = pool.admit(scns[2])
     assert s2 == s0                              # the freed slot is reused
     pool.advance(12, graph=False)
-    fin, act = pool.poll()
+    fin, act, _ = pool.poll()
     assert act == 2
 
 
@@ -159,7 +159,7 @@ def test_pool_retiles_when_it_drains(gpu, prec):
     assert pool.stats()["tiles"] == 16                  # now at 2 rows
     for _ in range(10000):
         pool.advance(6, graph=True)
-        fin, _ = pool.poll()
+        fin, _, _ = pool.poll()
         if ss in fin:
             break
     assert ss in fin
@@ -172,3 +172,75 @@ def test_pool_retiles_when_it_drains(gpu, prec):
     assert rp.shape == rd.shape and len(rp) > 13
     assert np.array_equal(rp[:, [0, 1, 3]], rd[:, [0, 1, 3]])
     np.testing.assert_allclose(rp[:, 2], rd[:, 2], rtol=1e-12, atol=0)
+
+
+def _poisoned(kind, seed, dx=10.0):
+    """A small scenario with finite inputs that goes bad on the device:
+    'inf'   a QFIX of 1e308 m^2/s (v*dx overflows: an infinite depth),
+    'nan'   that source and its negative on one cell (inf - inf = NaN, which
+            the H maximum never sees),
+    'stall' a QFIX of 1e300 (a finite 1e300 m depth, whose dt of ~1e-150 s
+            cannot move the FP64 clock past t = 10 s)."""
+    import dataclasses
+    from adaptflood_amd._lib import AF_BC_QFIX
+    from adaptflood_amd.scenario import PointBC
+    s = generators.small_case(48, 40, seed=seed, dx=dx, sim_time=200.0)
+    x, y = 20, 17
+    pts = list(s.points)
+    if kind == "inf":
+        pts.append(PointBC(x, y, AF_BC_QFIX, 1e308))
+    elif kind == "nan":
+        pts += [PointBC(x, y, AF_BC_QFIX, 1e308), PointBC(x, y, AF_BC_QFIX, -1e308)]
+    else:
+        pts.append(PointBC(x, y, AF_BC_QFIX, 1e300))
+    s.dem[y, x] = 1.0
+    return dataclasses.replace(s, points=pts, name=f"poisoned {kind}")
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_bad_scenarios_are_isolated_in_the_pool(gpu, prec):
+    """Served beside healthy scenarios: a scenario whose depth turns infinite,
+    one whose depth turns NaN, (FP64) one whose clock stalls, and one refused
+    at admission for an infinite initial depth each come back with an error;
+    the loop ends, and each healthy result equals its solo run bitwise."""
+    import dataclasses
+    from adaptflood_amd._lib import AF_SCN_NONFINITE, AF_SCN_STALLED, AFError
+    healthy = [generators.small_case(nx, ny, seed=150 + k, sim_time=80.0 + 20 * k)
+               for k, (nx, ny) in enumerate([(60, 41), (130, 70), (90, 33)])]
+    bad = [_poisoned("inf", 160), _poisoned("nan", 161)]
+    if prec == "fp64":
+        bad.append(_poisoned("stall", 162))
+    h0 = np.zeros_like(healthy[0].dem)
+    h0[3, 4] = np.inf
+    refused = dataclasses.replace(healthy[0], h0=h0, name="infinite h0")
+    order = [healthy[0], bad[0], healthy[1], refused, bad[1], healthy[2]] + bad[2:]
+    reqs = [Request(k, s, arrival=3 * k) for k, s in enumerate(order)]
+    pool = Pool(budget_bytes=8 << 20, precision=prec, max_scenarios=4, max_tiles=4096)
+    done, summary = serve(reqs, pool, chunk_steps=6, clock="steps", keep_results=True)
+    for r in reqs:
+        print(f"request {r.rid} ({r.scenario.name}): steps={r.steps} error={r.error}",
+              flush=True)
+    assert sorted(r.scenario.name for r in done) == sorted(s.name for s in healthy)
+    assert summary["rejected"] == 1 and summary["failed"] == len(bad)
+    by_name = {r.scenario.name: r for r in reqs}
+    assert "not finite" in by_name["infinite h0"].error
+    assert "NaN or infinite" in by_name["poisoned inf"].error
+    assert "NaN or infinite" in by_name["poisoned nan"].error
+    if prec == "fp64":
+        assert "stalled" in by_name["poisoned stall"].error
+    for r in done:
+        d = Domain(r.scenario, precision=prec)
+        d.run(check_every=12, graph=True)
+        assert r.steps == d.info()["steps"], r.rid
+        assert np.array_equal(r.result, d.read("H")), r.rid
+    # a lone domain reports the stop as an error, with its status
+    d = Domain(bad[1], precision=prec)
+    with pytest.raises(AFError, match="NaN or infinite"):
+        d.run(check_every=6, graph=False)
+    inf = d.info()
+    assert inf["status"] == AF_SCN_NONFINITE and inf["finished"] and inf["steps"] <= 2
+    if prec == "fp64":
+        d = Domain(bad[2], precision=prec)
+        with pytest.raises(AFError, match="stalled"):
+            d.run(check_every=6, graph=True)
+        assert d.info()["status"] == AF_SCN_STALLED and d.info()["steps"] == 1

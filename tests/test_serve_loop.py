@@ -1,13 +1,18 @@
 """The continuous-batching request loop (adaptflood_amd.serve.serve) on a
 stand-in pool that simulates admission capacity and step counts on the CPU:
 admission order, waiting while full, release and re-admission, completion
-times and the latency summary."""
-from adaptflood_amd.serve import Request, serve
+times and the latency summary, and the isolation of bad requests (refused at
+admission, stopped on the device, or with a clock that stalls)."""
+from adaptflood_amd._lib import AF_SCN_NONFINITE
+from adaptflood_amd.serve import Request, RequestError, serve
 
 
 class Scn:
-    def __init__(self, cells, steps):
+    def __init__(self, cells, steps, stall_at=None, fail_at=None, bad=False):
         self.cells, self.steps = cells, steps
+        self.stall_at = stall_at    # the clock stops moving after this many steps
+        self.fail_at = fail_at      # the device stops it (non-finite) after this many
+        self.bad = bad              # refused at admission
 
 
 class FakePool:
@@ -18,6 +23,8 @@ class FakePool:
         self.slots, self.active, self.log, self.step = slots, {}, [], 0
 
     def admit(self, scn):
+        if scn.bad:
+            raise RequestError("initial depth at column 0, row 0 is not finite (inf)")
         if len(self.active) >= self.slots:
             return None
         slot = min(set(range(self.slots)) - set(self.active))
@@ -32,14 +39,26 @@ class FakePool:
     def advance(self, steps, graph=True):
         self.step += steps
         for rec in self.active.values():
-            rec[1] = min(rec[0].steps, rec[1] + steps)
+            scn = rec[0]
+            end = scn.steps if scn.fail_at is None else min(scn.steps, scn.fail_at)
+            rec[1] = min(end, rec[1] + steps)
+
+    def _clock(self, scn, k):
+        # simulated time: one second per step, frozen past stall_at
+        return float(k if scn.stall_at is None else min(k, scn.stall_at))
+
+    def _status(self, scn, k):
+        return AF_SCN_NONFINITE if (scn.fail_at is not None and k >= scn.fail_at) else 0
 
     def poll(self):
-        done = [s for s, (scn, k) in sorted(self.active.items()) if k >= scn.steps]
-        return done, len(self.active)
+        done = [s for s, (scn, k) in sorted(self.active.items())
+                if k >= scn.steps or self._status(scn, k)]
+        clocks = {s: self._clock(scn, k) for s, (scn, k) in self.active.items()}
+        return done, len(self.active), clocks
 
     def info(self, slot):
-        return {"steps": self.active[slot][1]}
+        scn, k = self.active[slot]
+        return {"steps": k, "t": self._clock(scn, k), "status": self._status(scn, k)}
 
     def read(self, slot, what="H"):
         return ("result", self.active[slot][0].steps)
@@ -110,3 +129,25 @@ def test_request_larger_than_the_pool_is_rejected_not_waited_for():
     # while request 0 still held part of the pool, so request 2 waited behind
     # it until request 0 finished and request 1 was rejected
     assert reqs[2].admitted >= reqs[0].done
+
+
+def test_bad_requests_are_isolated_and_the_rest_complete():
+    """A request refused at admission, one the device stops (non-finite
+    depth) and one whose clock stalls each come back with an error; the loop
+    ends, and every healthy request runs its own number of steps."""
+    scns = [Scn(100, 30), Scn(100, 40, bad=True), Scn(100, 50, fail_at=12),
+            Scn(100, 60, stall_at=20), Scn(100, 25)]
+    reqs = [Request(k, s, arrival=k) for k, s in enumerate(scns)]
+    pool = FakePool(slots=3)
+    done, summary = serve(reqs, pool, chunk_steps=5, clock="steps", stall_polls=3)
+    assert sorted(r.rid for r in done) == [0, 4]
+    assert summary["requests"] == 2 and summary["rejected"] == 1 and summary["failed"] == 2
+    assert "refused" in reqs[1].error and "not finite" in reqs[1].error
+    assert reqs[1].admitted is None
+    assert "NaN or infinite" in reqs[2].error and reqs[2].steps == 12
+    assert "stalled" in reqs[3].error and reqs[3].steps > 20
+    assert all(r.error is None and r.steps == scns[r.rid].steps for r in done)
+    assert summary["cell_updates"] == 100 * (30 + 25)
+    # every admitted request was released, so the pool is empty again
+    assert not pool.active
+    assert sum(1 for e in pool.log if e[0] == "release") == 4
