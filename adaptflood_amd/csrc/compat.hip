@@ -172,6 +172,96 @@ k_apply_point_bcs(double* __restrict__ H, const int32_t* __restrict__ idx,
     }
 }
 
+// Point lists of up to kSortCap entries are grouped by cell with one sort in
+// LDS: key = cell << 32 | array position, so a stable order within a cell is
+// the array order.  The first entry of each run of equal cells then applies
+// the whole run, in array order.  O(n log^2 n) work in one workgroup, against
+// the n^2 compares of the scan kernels above, which remain for longer lists.
+constexpr int kSortCap = 8192;       // 64 KiB of keys in LDS
+constexpr int kSortThreads = 1024;
+constexpr unsigned long long kNoKey = ~0ull;
+
+// Sorts key[0, m) ascending (m a power of two) with a bitonic network; every
+// thread of the block takes part and the block is synchronised on return.
+__device__ void lds_bitonic_sort(unsigned long long* key, int m) {
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = key[i], b = key[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ int pow2_at_least(int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
+// AddFlowBCs (main.cpp:256-260) for n <= kSortCap sources.
+__global__ void __launch_bounds__(kSortThreads)
+k_add_flow_sorted(double* __restrict__ H, const int32_t* __restrict__ idx,
+                  const double* __restrict__ ps, int n) {
+    __shared__ unsigned long long key[kSortCap];
+    const int m = pow2_at_least(n);
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        const int c = (k < n) ? idx[k] : -1;
+        key[k] = (c >= 0) ? ((unsigned long long)c << 32) | (unsigned)k : kNoKey;
+    }
+    __syncthreads();
+    lds_bitonic_sort(key, m);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const unsigned long long e = key[k];
+        if (e == kNoKey) continue;
+        const unsigned c = (unsigned)(e >> 32);
+        if (k > 0 && (unsigned)(key[k - 1] >> 32) == c) continue;  // not the run's first
+        double h = H[c];
+        for (int r = k; r < n && key[r] != kNoKey && (unsigned)(key[r] >> 32) == c; r++) {
+            h += ps[(unsigned)key[r]];
+            if (h < 0.0) h = 0.0;
+        }
+        H[c] = h;
+    }
+}
+
+// ApplyPointBCs (kernels.cpp:261-274) for n <= kSortCap points: the last
+// FREE/HFIX/HVAR point of each cell in array order wins.
+__global__ void __launch_bounds__(kSortThreads)
+k_apply_point_bcs_sorted(double* __restrict__ H, const int32_t* __restrict__ idx,
+                         const int32_t* __restrict__ type, const double* __restrict__ value,
+                         int n) {
+    __shared__ unsigned long long key[kSortCap];
+    const int m = pow2_at_least(n);
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        int c = -1;
+        if (k < n) {
+            const int ty = type[k];
+            if (ty == 1 || ty == 2 || ty == 3) c = idx[k];
+        }
+        key[k] = (c >= 0) ? ((unsigned long long)c << 32) | (unsigned)k : kNoKey;
+    }
+    __syncthreads();
+    lds_bitonic_sort(key, m);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const unsigned long long e = key[k];
+        if (e == kNoKey) continue;
+        const unsigned c = (unsigned)(e >> 32);
+        if (k + 1 < n && key[k + 1] != kNoKey && (unsigned)(key[k + 1] >> 32) == c) continue;
+        const int b = (int)(unsigned)e;   // the run's last entry
+        H[c] = (type[b] == 1) ? 0.0 : value[b];
+    }
+}
+
 // UpdateH (kernels.cpp:281-288)
 __global__ void __launch_bounds__(kThreads)
 k_update_h(double* __restrict__ H, const double* __restrict__ Qx, const double* __restrict__ Qy,
@@ -261,6 +351,36 @@ int check_grid(int xsz, int ysz) {
     return AF_OK;
 }
 
+// FloodplainQ_GPU with the H-max key at `key` (kernels.cpp:22-146).
+int floodplain_q(const double* H, const double* DEM, double* Qx, double* Qy, const double* Qxold,
+                 const double* Qyold, int xsz, int ysz, double dx, double FPn, double DepthThresh,
+                 double MaxHflow, double* Tstep, double cfl, double g, double InitTstep,
+                 double theta, double nodata, unsigned long long* key, hipStream_t s) {
+    long long n = (long long)xsz * ysz;
+    // the key is cleared by a kernel, not a memset, so the call sequence
+    // holds only kernel launches and one read-back
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, key, fkey(-DBL_MAX));
+    unsigned hb = blocks_for(n, kThreads);
+    if (hb > 4096) hb = 4096;
+    hipLaunchKernelGGL(k_hmax, dim3(hb), dim3(kThreads), 0, s, H, DEM, n, nodata, key);
+    AF_HIP(hipGetLastError());
+    unsigned long long hkey = 0;
+    AF_HIP(hipMemcpyAsync(&hkey, key, sizeof(hkey), hipMemcpyDeviceToHost, s));
+    AF_HIP(hipStreamSynchronize(s));
+    double Hmax = fkey_inv_d(hkey);
+    double dt = cfl_dt(Hmax, DepthThresh, cfl, dx, g, InitTstep);
+    *Tstep = dt;
+    FluxConstsF64 k{dx, FPn, DepthThresh, MaxHflow, g * dt, theta};
+    long long nx = (long long)(xsz - 1) * ysz, ny = (long long)xsz * (ysz - 1);
+    hipLaunchKernelGGL(k_qx, dim3(blocks_for(nx, kThreads)), dim3(kThreads), 0, s, H, DEM, Qx,
+                       Qxold, Qyold, xsz, ysz, nodata, k);
+    hipLaunchKernelGGL(k_qy, dim3(blocks_for(ny, kThreads)), dim3(kThreads), 0, s, H, DEM, Qy,
+                       Qxold, Qyold, xsz, ysz, nodata, k);
+    AF_HIP(hipGetLastError());
+    AF_HIP(hipStreamSynchronize(s));
+    return AF_OK;
+}
+
 }  // namespace
 }  // namespace af
 
@@ -294,29 +414,9 @@ int af_FloodplainQ_GPU(const double* H, const double* DEM, double* Qx, double* Q
     if (rc) return rc;
     AF_REQUIRE(H && DEM && Qx && Qy && Qxold && Qyold && Tstep && workspace, AF_E_INVALID,
                "FloodplainQ: null pointer");
-    hipStream_t s = to_stream(stream);
-    unsigned long long* key = reinterpret_cast<unsigned long long*>(workspace);
-    long long n = (long long)xsz * ysz;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, s, key, fkey(-DBL_MAX));
-    unsigned hb = blocks_for(n, kThreads);
-    if (hb > 4096) hb = 4096;
-    hipLaunchKernelGGL(k_hmax, dim3(hb), dim3(kThreads), 0, s, H, DEM, n, nodata, key);
-    AF_HIP(hipGetLastError());
-    unsigned long long hkey = 0;
-    AF_HIP(hipMemcpyAsync(&hkey, key, sizeof(hkey), hipMemcpyDeviceToHost, s));
-    AF_HIP(hipStreamSynchronize(s));
-    double Hmax = fkey_inv_d(hkey);
-    double dt = cfl_dt(Hmax, DepthThresh, cfl, dx, g, InitTstep);
-    *Tstep = dt;
-    FluxConstsF64 k{dx, FPn, DepthThresh, MaxHflow, g * dt, theta};
-    long long nx = (long long)(xsz - 1) * ysz, ny = (long long)xsz * (ysz - 1);
-    hipLaunchKernelGGL(k_qx, dim3(blocks_for(nx, kThreads)), dim3(kThreads), 0, s, H, DEM, Qx,
-                       Qxold, Qyold, xsz, ysz, nodata, k);
-    hipLaunchKernelGGL(k_qy, dim3(blocks_for(ny, kThreads)), dim3(kThreads), 0, s, H, DEM, Qy,
-                       Qxold, Qyold, xsz, ysz, nodata, k);
-    AF_HIP(hipGetLastError());
-    AF_HIP(hipStreamSynchronize(s));
-    return AF_OK;
+    return floodplain_q(H, DEM, Qx, Qy, Qxold, Qyold, xsz, ysz, dx, FPn, DepthThresh, MaxHflow,
+                        Tstep, cfl, g, InitTstep, theta, nodata,
+                        reinterpret_cast<unsigned long long*>(workspace), to_stream(stream));
 }
 
 int af_BCs_GPU(double* Qx, double* Qy, const double* H, const double* DEM, int xsz, int ysz,
@@ -344,8 +444,12 @@ int af_ApplyPointBCs_GPU(double* H, const int32_t* bc_idx, const int32_t* bc_typ
     if (num_bcs == 0) return AF_OK;
     AF_REQUIRE(H && bc_idx && bc_type && bc_value, AF_E_INVALID, "ApplyPointBCs: null pointer");
     hipStream_t s = to_stream(stream);
-    hipLaunchKernelGGL(k_apply_point_bcs, dim3(blocks_for(num_bcs, kThreads)), dim3(kThreads), 0,
-                       s, H, bc_idx, bc_type, bc_value, num_bcs);
+    if (num_bcs <= kSortCap)
+        hipLaunchKernelGGL(k_apply_point_bcs_sorted, dim3(1), dim3(kSortThreads), 0, s, H, bc_idx,
+                           bc_type, bc_value, num_bcs);
+    else
+        hipLaunchKernelGGL(k_apply_point_bcs, dim3(blocks_for(num_bcs, kThreads)), dim3(kThreads),
+                           0, s, H, bc_idx, bc_type, bc_value, num_bcs);
     AF_HIP(hipGetLastError());
     AF_HIP(hipStreamSynchronize(s));
     return AF_OK;
@@ -387,8 +491,12 @@ int af_AddFlowBCs_GPU(double* H, const int32_t* ps_idx, const double* ps_flow, i
     if (num_flow_bcs == 0) return AF_OK;
     AF_REQUIRE(H && ps_idx && ps_flow, AF_E_INVALID, "AddFlowBCs: null pointer");
     hipStream_t s = to_stream(stream);
-    hipLaunchKernelGGL(k_add_flow, dim3(blocks_for(num_flow_bcs, kThreads)), dim3(kThreads), 0, s,
-                       H, ps_idx, ps_flow, num_flow_bcs);
+    if (num_flow_bcs <= kSortCap)
+        hipLaunchKernelGGL(k_add_flow_sorted, dim3(1), dim3(kSortThreads), 0, s, H, ps_idx,
+                           ps_flow, num_flow_bcs);
+    else
+        hipLaunchKernelGGL(k_add_flow, dim3(blocks_for(num_flow_bcs, kThreads)), dim3(kThreads), 0,
+                           s, H, ps_idx, ps_flow, num_flow_bcs);
     AF_HIP(hipGetLastError());
     AF_HIP(hipStreamSynchronize(s));
     return AF_OK;

@@ -123,3 +123,29 @@ def test_sequence_plane_case(gpu):
     err = float(np.max(np.abs(out["H"] - ref["H"])))
     print(f"compat sequence plane 600 s: steps={out['steps']} max|dH|={err:.3e}", flush=True)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("n", [1, 2, 5000, 8192, 9000])
+def test_point_lists_with_duplicates(gpu, n):
+    """AddFlowBCs_GPU and ApplyPointBCs_GPU on long lists (up to 8192 entries
+    grouped by one sort in LDS, longer ones by the scan kernels), with cells
+    repeated up to five times and negative depths to clamp: bitwise the
+    oracle's sequential file-order loops (main.cpp:256-260, kernels.cpp:261-274)."""
+    nx, ny = 300, 211
+    rng = np.random.default_rng(1000 + n)
+    dem, H, _, _ = _state(1000 + n, nx, ny)
+    cells = rng.integers(0, nx * ny, max(1, n // 3))
+    idx = rng.choice(cells, n).astype(np.int32)          # many duplicates
+    ps = rng.normal(0.0, 0.3, n)
+    Href = H.copy()
+    oracle.add_flow_bcs(Href, idx, ps)
+    gH = _dev(H, gpu)
+    K.AddFlowBCs_GPU(gH, _dev(idx, gpu), _dev(ps, gpu), n)
+    assert np.array_equal(gH.cpu().numpy(), Href)
+    code = rng.integers(1, 4, n).astype(np.int32)
+    code[rng.random(n) < 0.1] = 4                         # not an override: ignored
+    val = rng.uniform(0.0, 2.0, n)
+    oracle.apply_point_bcs(Href, idx, code, val)
+    K.ApplyPointBCs_GPU(gH, _dev(idx, gpu), _dev(code, gpu), _dev(val, gpu), n, nx, 5.0, 25.0,
+                        1.0)
+    assert np.array_equal(gH.cpu().numpy(), Href)
