@@ -229,3 +229,33 @@ def test_fused_console_matches_reference_sequence(gpu, tmp_path):
     steps = [ln for ln in lines["compat"] if ln.startswith("Step ")]
     assert len(steps) > 12 and steps[10].startswith("Step 100,")
     assert lines["fp64"] == lines["compat"]
+
+
+def test_every_step_saves_and_reports(gpu, tmp_path):
+    """saveint 0 and massint 0: next_save and next_mass never pass t, so the
+    reference saves and reports after every step (main.cpp:291-304,
+    326-363).  The fused FP64 path writes the same H files and prints the
+    same report lines as the compat path (the reference call sequence)."""
+    import shutil
+    scn = generators.small_case(30, 20, seed=41, sim_time=60.0)
+    scn.mass_int = 0.0
+    par = formats.write_case(str(tmp_path / "c"), scn, extra_par="saveint 0\nasync_output 0\n")
+    res = str(tmp_path / "c" / "results")
+    runs = {}
+    for path in ("compat", "fp64"):
+        r = subprocess.run([EXE, "--path", path, "--quiet", par], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        reps = [ln for ln in r.stdout.splitlines() if REPORT.match(ln)]
+        files = _saves(res, "H")
+        grids = {f: _read_asc(os.path.join(res, f))[1] for f in files}
+        runs[path] = (reps, files, grids, r.stdout)
+        shutil.rmtree(res)
+    (rc, fc, gc, oc), (rf, ff, gf, of) = runs["compat"], runs["fp64"]
+    steps = int(re.search(r"Steps: (\d+)", oc).group(1))
+    print(f"every-step run: {steps} steps, {len(fc)} saves, {len(rc)} reports", flush=True)
+    assert steps > 5 and len(rc) == steps and len(fc) == steps + 1
+    assert rf == rc and ff == fc
+    assert "not recorded" not in of
+    for f in fc:
+        assert np.array_equal(gf[f], gc[f]), f
