@@ -81,6 +81,8 @@ SIGNATURES = [
     ("af_UpdateQs_GPU", _I, [_V, _V, _V, _V, _I, _I, _D, _V]),
     ("af_FloodplainQ_GPU", _I, [_V, _V, _V, _V, _V, _V, _I, _I, _D, _D, _D, _D, _D, _D, _dp,
                                 _D, _D, _D, _D, _D, _D, _D, _V, _V]),
+    ("af_FloodplainQ_GPU_noalloc", _I, [_V, _V, _V, _V, _V, _V, _I, _I, _D, _D, _D, _D, _D, _D,
+                                        _dp, _D, _D, _D, _D, _D, _D, _D, _V]),
     ("af_BCs_GPU", _I, [_V, _V, _V, _V, _I, _I, _D, _D, _D, _D, _I, _I, _I, _I, _V]),
     ("af_ApplyPointBCs_GPU", _I, [_V, _V, _V, _V, _I, _I, _D, _D, _D, _V]),
     ("af_UpdateH_GPU", _I, [_V, _V, _V, _I, _I, _D, _D, _V]),

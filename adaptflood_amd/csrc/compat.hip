@@ -6,6 +6,8 @@
 // af_device.hpp.
 #include <float.h>
 
+#include <atomic>
+
 #include "af_device.hpp"
 #include "af_host.hpp"
 
@@ -351,6 +353,43 @@ int check_grid(int xsz, int ysz) {
     return AF_OK;
 }
 
+// Scratch of af_FloodplainQ_GPU_noalloc: the H-max key of one call (one
+// 64-bit word, padded to the workspace size of af_FloodplainQ_GPU).  Each GPU
+// the library runs on holds its own copy until the process ends.  Host code
+// never passes `scratch` itself to a kernel (on the host it names a shadow
+// variable, not device memory): it passes the device address that
+// hipGetSymbolAddress returned for the current device.
+constexpr int kScratchBytes = AF_FLOODPLAINQ_WORKSPACE_BYTES;
+static __device__ __align__(256) unsigned long long scratch[kScratchBytes / 8];
+
+constexpr int kMaxDevices = 64;
+std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the device `stream` runs on (the current
+// device for the null stream), looked up once per device with that device
+// current.
+int scratch_ptr(hipStream_t stream, unsigned long long** out) {
+    int cur = 0, dev = 0;
+    AF_HIP(hipGetDevice(&cur));
+    dev = cur;
+    if (stream) AF_HIP(hipStreamGetDevice(stream, &dev));
+    AF_REQUIRE(dev >= 0 && dev < kMaxDevices, AF_E_INVALID, "device %d out of range", dev);
+    void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        if (dev != cur) AF_HIP(hipSetDevice(dev));
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur) AF_HIP(hipSetDevice(cur));
+        if (e != hipSuccess) {
+            set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+            return AF_E_HIP;
+        }
+        AF_REQUIRE(p != nullptr, AF_E_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+        g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+    *out = reinterpret_cast<unsigned long long*>(p);
+    return AF_OK;
+}
+
 // FloodplainQ_GPU with the H-max key at `key` (kernels.cpp:22-146).
 int floodplain_q(const double* H, const double* DEM, double* Qx, double* Qy, const double* Qxold,
                  const double* Qyold, int xsz, int ysz, double dx, double FPn, double DepthThresh,
@@ -417,6 +456,24 @@ int af_FloodplainQ_GPU(const double* H, const double* DEM, double* Qx, double* Q
     return floodplain_q(H, DEM, Qx, Qy, Qxold, Qyold, xsz, ysz, dx, FPn, DepthThresh, MaxHflow,
                         Tstep, cfl, g, InitTstep, theta, nodata,
                         reinterpret_cast<unsigned long long*>(workspace), to_stream(stream));
+}
+
+int af_FloodplainQ_GPU_noalloc(const double* H, const double* DEM, double* Qx, double* Qy,
+                               const double* Qxold, const double* Qyold, int xsz, int ysz,
+                               double dx, double dy, double FPn, double DepthThresh,
+                               double MaxHflow, double dhlin, double* Tstep, double Qlimfact,
+                               double dA, double cfl, double g, double InitTstep, double theta,
+                               double nodata, af_stream_t stream) {
+    (void)dy; (void)dhlin; (void)Qlimfact; (void)dA;
+    int rc = check_grid(xsz, ysz);
+    if (rc) return rc;
+    AF_REQUIRE(H && DEM && Qx && Qy && Qxold && Qyold && Tstep, AF_E_INVALID,
+               "FloodplainQ: null pointer");
+    hipStream_t s = to_stream(stream);
+    unsigned long long* key = nullptr;
+    if ((rc = scratch_ptr(s, &key))) return rc;
+    return floodplain_q(H, DEM, Qx, Qy, Qxold, Qyold, xsz, ysz, dx, FPn, DepthThresh, MaxHflow,
+                        Tstep, cfl, g, InitTstep, theta, nodata, key, s);
 }
 
 int af_BCs_GPU(double* Qx, double* Qy, const double* H, const double* DEM, int xsz, int ysz,

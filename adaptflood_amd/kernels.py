@@ -80,6 +80,21 @@ def FloodplainQ_GPU(H, DEM, Qx, Qy, Qxold, Qyold, xsz, ysz, dx, dy, FPn, DepthTh
     return ts.value
 
 
+def FloodplainQ_GPU_noalloc(H, DEM, Qx, Qy, Qxold, Qyold, xsz, ysz, dx, dy, FPn, DepthThresh,
+                            MaxHflow, dhlin, Tstep, Qlimfact, dA, cfl, g, InitTstep, theta,
+                            nodata, stream=None):
+    """FloodplainQ_GPU with its H-max key in the library's static device
+    scratch instead of a workspace (no counterpart in the reference; for
+    callers that cannot allocate).  Calls on one GPU must not overlap."""
+    _f64(H, DEM, Qx, Qy, Qxold, Qyold)
+    ts = C.c_double(float(Tstep))
+    call("af_FloodplainQ_GPU_noalloc", _p(H), _p(DEM), _p(Qx), _p(Qy), _p(Qxold), _p(Qyold),
+         int(xsz), int(ysz), float(dx), float(dy), float(FPn), float(DepthThresh),
+         float(MaxHflow), float(dhlin), C.byref(ts), float(Qlimfact), float(dA), float(cfl),
+         float(g), float(InitTstep), float(theta), float(nodata), _s(stream))
+    return ts.value
+
+
 def BCs_GPU(Qx, Qy, H, DEM, xsz, ysz, dx, g, dt, nodata, west_free, east_free, north_free,
             south_free, stream=None):
     """kernels.cpp:149-256"""

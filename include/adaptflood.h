@@ -24,12 +24,15 @@
  *
  * Every entry point returns an int status: AF_OK (0) or a negative AF_E_*
  * code; af_last_error() gives a message for the calling thread.  The library
- * never allocates device memory: the caller owns every device buffer.
+ * never allocates device memory: the caller owns every device buffer.  The
+ * one exception is a 256-byte static device array per GPU, the scratch of
+ * af_FloodplainQ_GPU_noalloc, which the loader places with the code.
  *
  * Threads: the per-function entry points keep no state between calls and
  * may be called from several threads at once (each with its own buffers and
- * FloodplainQ workspace).  A domain, batch or pool (af_domain) is not
- * thread-safe: calls on one af_domain must not overlap.
+ * FloodplainQ workspace), except af_FloodplainQ_GPU_noalloc, whose calls on
+ * one GPU share its scratch and must not overlap.  A domain, batch or pool
+ * (af_domain) is not thread-safe: calls on one af_domain must not overlap.
  */
 #ifndef ADAPTFLOOD_AMD_H
 #define ADAPTFLOOD_AMD_H
@@ -80,6 +83,24 @@ int af_FloodplainQ_GPU(const double* H, const double* DEM, double* Qx, double* Q
                        double* Tstep, double Qlimfact, double dA, double cfl, double g,
                        double InitTstep, double theta, double nodata, void* workspace,
                        af_stream_t stream);
+
+/* af_FloodplainQ_GPU without the workspace argument, and allocating nothing:
+ * no hipMalloc, hipFree or allocating library call.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture elsewhere in their loop, serving loops, harnesses
+ * that forbid allocation).  The H-max key lives in a static device array of
+ * AF_FLOODPLAINQ_WORKSPACE_BYTES (256 bytes) that each GPU holds until the
+ * process ends.  The scratch need is one 64-bit key whatever the grid, so
+ * every grid the other arguments can describe fits it.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two threads, or two
+ * streams) race and may return each other's Tstep.  Each call clears the key
+ * before it reduces into it.  Results equal af_FloodplainQ_GPU's bitwise. */
+int af_FloodplainQ_GPU_noalloc(const double* H, const double* DEM, double* Qx, double* Qy,
+                               const double* Qxold, const double* Qyold, int xsz, int ysz,
+                               double dx, double dy, double FPn, double DepthThresh,
+                               double MaxHflow, double dhlin, double* Tstep, double Qlimfact,
+                               double dA, double cfl, double g, double InitTstep, double theta,
+                               double nodata, af_stream_t stream);
 
 /* Replaces BCs_GPU (src/kernels.hpp:19-21, src/kernels.cpp:149-256): FREE or
  * CLOSED edge faces from the pre-update H.  dt is unused, as in the reference. */
