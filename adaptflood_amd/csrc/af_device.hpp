@@ -146,7 +146,19 @@ struct FluxConstsF32 {
     float gdt_n2;   // g*dt*n^2: den = 1 + g*dt*n^2*|q|*hflow^(-7/3)
     float sc;       // 1/dx: Q -> q_old
     float sc4;      // 0.25/dx: sum of four Q -> transverse q_old average
+    float zsc;      // metres per unit of the fixed-point terrain (a power of two)
 };
+
+// Terrain step z0 - z1 of a face from the FP32 path's fixed-point terrain
+// (integer units of zsc metres).  The integer difference is exact (taken
+// modulo 2^32, which no valid pair reaches; a NODATA pair's value is never
+// used), so dz carries one FP32 rounding of the step instead of the
+// rounding of each elevation: stored as float, an elevation 50 m above the
+// lowest cell is known to about 4e-6 m, as fixed point to zsc/2 (6e-8 m for
+// 80 m of relief).
+__device__ __forceinline__ float terrain_step(int z0, int z1, float zsc) {
+    return (float)(int)((unsigned)z0 - (unsigned)z1) * zsc;
+}
 
 // Surface step and flow depth of a face in FP32 without forming the surface
 // elevations z + h: tens of metres of terrain round to about 1e-6 m in FP32,
@@ -154,9 +166,8 @@ struct FluxConstsF32 {
 //   dh    = (z0 + h0) - (z1 + h1)          = dz + (h0 - h1)
 //   hflow = max(z0+h0, z1+h1) - max(z0, z1) = dz >= 0 ? max(h0, h1 - dz)
 //                                                     : max(h0 + dz, h1)
-__device__ __forceinline__ void face_depths_f32(float z0, float z1, float h0, float h1, float& dh,
+__device__ __forceinline__ void face_depths_f32(float dz, float h0, float h1, float& dh,
                                                 float& hflow) {
-    const float dz = z0 - z1;
     dh = dz + (h0 - h1);
     hflow = (dz >= 0.0f) ? getmax(h0, h1 - dz) : getmax(h0 + dz, h1);
 }
@@ -170,10 +181,10 @@ __device__ __forceinline__ void face_depths_f32(float z0, float z1, float h0, fl
 // branch per face costs more exec-mask instructions than the formula, and a
 // lone wavefront's row is bound by the instructions it issues.  (On a dry
 // lane the discarded value may be Inf or NaN; it is never used.)
-__device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, float h1, float q0,
-                                               float qt, const FluxConstsF32& k) {
+__device__ __forceinline__ float face_flux_f32(float dz, float h0, float h1, float q0, float qt,
+                                               const FluxConstsF32& k) {
     float dh, hflow;
-    face_depths_f32(z0, z1, h0, h1, dh, hflow);
+    face_depths_f32(dz, h0, h1, dh, hflow);
     hflow = getmax(hflow, 0.0f);
     hflow = getmin(hflow, k.maxh);
     const float qvect = __builtin_amdgcn_sqrtf(q0 * q0 + qt * qt);
@@ -185,12 +196,12 @@ __device__ __forceinline__ float face_flux_f32(float z0, float z1, float h0, flo
 }
 
 // face_flux_f32 on stored discharges (see face_flux_ref_t1_q).
-__device__ __forceinline__ float face_flux_f32_q(float z0, float z1, float h0, float h1, float Q0,
-                                                 float a, float b, float c, float d,
+__device__ __forceinline__ float face_flux_f32_q(float dz, float h0, float h1, float Q0, float a,
+                                                 float b, float c, float d,
                                                  const FluxConstsF32& k) {
     const float q0 = Q0 * k.sc;
     const float qt = (((a + b) + c) + d) * k.sc4;
-    return face_flux_f32(z0, z1, h0, h1, q0, qt, k);
+    return face_flux_f32(dz, h0, h1, q0, qt, k);
 }
 
 // Edge fluxes of src/kernels.cpp:154-253 from the pre-update depth.
@@ -206,6 +217,20 @@ __device__ __forceinline__ T edge_west(T z0, T h0, T z1, T h1, bool nodata0, T d
     }
     return T(0);
 }
+// edge_west in FP32 from the terrain step dz10 = z1 - z0: the surface step
+// y1 - y0 = dz10 + (h1 - h0), and y0 > z0 holds for every h0 >= 0.001 (the
+// elevation is finite), so the test reduces to h0 > 0.
+__device__ __forceinline__ float edge_west_f32(float h0, float h1, float dz10, bool nodata0,
+                                               float dx, float g) {
+    if (nodata0 || h0 < 0.001f) return 0.0f;
+    if (h0 > 0.0f) {
+        const float Sf = (dz10 + (h1 - h0)) / dx;
+        if (Sf < 0.0f) return -h0 * sqrtf(g * h0) * dx * 0.5f;
+        return 0.0f;
+    }
+    return 0.0f;
+}
+
 // East and South (kernels.cpp:188-202, 234-248): outflow +h0*sqrt(g*h0)*dx/2.
 template <typename T>
 __device__ __forceinline__ T edge_out_pos(T h0, bool nodata0, T dx, T g) {

@@ -14,8 +14,6 @@
 namespace af {
 namespace {
 
-constexpr float kNodataF32 = -1.0e30f;  // z sentinel of NODATA cells at FP32
-
 inline unsigned long long host_key(double x) {
     unsigned long long b;
     memcpy(&b, &x, 8);
@@ -41,7 +39,7 @@ struct HostScn {
     int R = 0;                   // tile rows of this scenario
     int R_alone = 0;             // tile rows it would take running alone (<= R)
     int cap_tiles = 0;           // tiles at R_alone: the per-tile arrays are sized for it
-    double zmin = 0.0;
+    double zbase = 0.0, zscale = 1.0;   // fixed-point terrain of the FP32 path
     // offsets in the arena
     size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
     size_t o_src_begin, o_src_j, o_src_i, o_src_val, o_src_off, o_src_len;
@@ -407,28 +405,47 @@ size_t heap_free_bytes(const af_domain* D) {
     return n;
 }
 
-// The DEM and the initial depths in the device layout and precision.  Returns
+// The FP32 path keeps the terrain as 32-bit fixed point: integer units of
+// the smallest power-of-two step 2^k (metres) that spans the valid cells in
+// at most kTerrainUnits units, above a base that is a multiple of the step.
+// Elevations on any grid of 2^-m metres (m <= -k) are exact; others round to
+// half a step (3e-8 m for 80 m of relief, where a float elevation would round
+// to 4e-6 m).
+constexpr double kTerrainUnits = 2147483000.0;   // below INT_MAX; INT_MIN is NODATA
+
+void terrain_scale(const af_scenario_desc* d, double& zbase, double& zscale) {
+    bool any = false;
+    double lo = 0.0, hi = 0.0;
+    for (long long c = 0; c < (long long)d->nx * d->ny; c++) {
+        const double v = d->dem[c];
+        if (v == d->nodata || !std::isfinite(v)) continue;
+        if (!any || v < lo) lo = v;
+        if (!any || v > hi) hi = v;
+        any = true;
+    }
+    int k = -40;   // a flat or empty terrain keeps a fine step
+    if (any && hi > lo) k = std::max(k, (int)std::ceil(std::log2((hi - lo) / kTerrainUnits)));
+    for (;; k++) {
+        zscale = std::ldexp(1.0, k);
+        zbase = any ? std::floor(lo / zscale) * zscale : 0.0;
+        if (!any || (hi - zbase) / zscale <= kTerrainUnits) break;
+    }
+}
+
+// The DEM and the initial depths in the device layout and precision (the
+// terrain as Z: double elevations at FP64, fixed point at FP32).  Returns
 // AF_E_INVALID for a non-finite DEM value (other than a non-finite NODATA
 // value itself) or initial depth: such a cell would turn its neighbours'
 // fluxes to NaN on the first step.
-template <typename T>
-int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>& z,
-                   std::vector<T>& H, double& zmin, double& hmax) {
+template <typename T, typename Z>
+int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<Z>& z,
+                   std::vector<T>& H, double& zbase, double& zscale, double& hmax) {
     const long long P = h.P;
-    const bool f64 = sizeof(T) == 8;
-    zmin = 0.0;
-    if (!f64) {
-        bool any = false;
-        double m = 0.0;
-        for (long long c = 0; c < (long long)d->nx * d->ny; c++) {
-            double v = d->dem[c];
-            if (v == d->nodata) continue;
-            if (!any || v < m) m = v;
-            any = true;
-        }
-        zmin = any ? m : 0.0;
-    }
-    z.assign((size_t)P * d->ny, T(0));
+    constexpr bool f64 = sizeof(T) == 8;
+    zbase = 0.0;
+    zscale = 1.0;
+    if (!f64) terrain_scale(d, zbase, zscale);
+    z.assign((size_t)P * d->ny, Z(0));
     H.assign((size_t)P * d->ny, T(0));
     hmax = -DBL_MAX;
     for (int j = 0; j < d->ny; j++) {
@@ -436,12 +453,12 @@ int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<T>& 
             const long long c = i + (long long)j * d->nx;
             const double v = d->dem[c];
             const bool nod = (v == d->nodata);
-            T zt;
-            if (f64) zt = (T)v;
-            else zt = nod ? (T)kNodataF32 : (T)(v - zmin);
-            z[(size_t)j * P + i] = zt;
             AF_REQUIRE(nod || std::isfinite(v), AF_E_INVALID,
                        "DEM value at column %d, row %d is not finite (%g)", i, j, v);
+            Z zt;
+            if constexpr (f64) zt = (Z)v;
+            else zt = nod ? (Z)INT_MIN : (Z)std::llround((v - zbase) / zscale);
+            z[(size_t)j * P + i] = zt;
             T ht = d->h0 ? (T)d->h0[c] : T(0);
             AF_REQUIRE(std::isfinite((double)ht), AF_E_INVALID,
                        "initial depth at column %d, row %d is not finite (%g)", i, j,
@@ -486,7 +503,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     v.theta = d->theta;
     v.sim_time = d->sim_time;
     v.mass_int = d->mass_int;
-    v.nodata_t = D->precision == AF_FP64 ? d->nodata : (double)kNodataF32;
+    v.nodata_t = d->nodata;   // the FP32 terrain marks NODATA with INT_MIN instead
     v.west_free = d->west_free ? 1 : 0;
     v.east_free = d->east_free ? 1 : 0;
     v.north_free = d->north_free ? 1 : 0;
@@ -530,19 +547,22 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     double hmax = -DBL_MAX;
     if (D->precision == AF_FP64) {
         std::vector<double> z, H;
-        int crc = convert_inputs<double>(d, h, z, H, h.zmin, hmax);
+        int crc = convert_inputs<double, double>(d, h, z, H, h.zbase, h.zscale, hmax);
         if (crc) return crc;
         AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipStreamSynchronize(st));
     } else {
-        std::vector<float> z, H;
-        int crc = convert_inputs<float>(d, h, z, H, h.zmin, hmax);
+        std::vector<float> H;
+        std::vector<int> z;
+        int crc = convert_inputs<float, int>(d, h, z, H, h.zbase, h.zscale, hmax);
         if (crc) return crc;
         AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
         AF_HIP(hipStreamSynchronize(st));
     }
+    v.zscale = h.zscale;
+    v.zbase = h.zbase;
     auto up = [&](size_t off, const void* src, size_t bytes) -> int {
         if (bytes == 0) return AF_OK;
         AF_HIP(hipMemcpyAsync(A + off, src, bytes, hipMemcpyHostToDevice, st));
@@ -731,7 +751,7 @@ int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
 // A slot is finished when its clock reached sim_time (main.cpp:227) or a
 // non-zero status stopped it.
 bool slot_finished(const af_domain* D, int s, const Ctl& c, int par) {
-    return !(c.t[par] < D->scn[s].dev.sim_time) || c.status != 0;
+    return !(c.t[par] < D->scn[s].dev.sim_time) || c.status[par] != 0;
 }
 
 const char* status_text(int status) {
@@ -1120,7 +1140,7 @@ int af_pool_progress(af_domain* D, int32_t* slots, double* t, int64_t* steps, in
         if (slots) slots[a] = s;
         if (t) t[a] = c[a].t[par];
         if (steps) steps[a] = c[a].n_done;
-        if (status) status[a] = c[a].status;
+        if (status) status[a] = c[a].status[par];
         if (finished) finished[a] = slot_finished(D, s, c[a], par) ? 1 : 0;
     }
     *n = na;
@@ -1205,9 +1225,9 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
             // every scenario ran to its sim_time or stopped; name the first
             // that stopped
             for (size_t a = 0; a < D->active.size(); a++) {
-                if (c[a].status == 0) continue;
+                if (c[a].status[par] == 0) continue;
                 set_error("scenario %d stopped after step %lld at t = %.17g: %s", D->active[a],
-                          (long long)c[a].n_done, c[a].t[par], status_text(c[a].status));
+                          (long long)c[a].n_done, c[a].t[par], status_text(c[a].status[par]));
                 return AF_E_STATE;
             }
             break;
@@ -1254,7 +1274,7 @@ int af_batch_info(af_domain* D, int s, af_run_info* info, af_stream_t stream) {
     info->n_reports = c.n_reports;
     info->finished = slot_finished(D, s, c, tpar) ? 1 : 0;
     info->hmax = fkey_inv_d(c.hmax[(int)(D->k % 3)]);
-    info->status = c.status;
+    info->status = c.status[tpar];
     info->reserved = 0;
     return AF_OK;
 }
@@ -1359,10 +1379,12 @@ k_snapshot(const ScnDev* __restrict__ scn, int kpar, double* __restrict__ Ho,
     const ScnDev& S = *scn;
     const int b = S.ctl->par[kpar];
     const T* __restrict__ H = static_cast<const T*>(S.H[b]);
-    const T* __restrict__ z = static_cast<const T*>(S.z);
+    const Zt<T>* __restrict__ z = static_cast<const Zt<T>*>(S.z);
     const T* __restrict__ Qx = static_cast<const T*>(S.Qx[b]);
     const T* __restrict__ Qy = static_cast<const T*>(S.Qy[b]);
-    const T nod = (T)S.nodata_t;
+    Zt<T> nod;
+    if constexpr (sizeof(T) == 8) nod = S.nodata_t;
+    else nod = INT_MIN;
     const double dx = S.dx;
     const long long n = (long long)S.nx * S.ny;
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;

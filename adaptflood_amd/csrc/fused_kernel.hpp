@@ -40,7 +40,12 @@ struct Ctl {
     double pending_t, pending_dt;
     int n_reports;
     int par[2];                   // buffer holding the state entering step k: par[k & 1]
-    int status;                   // AF_SCN_* bits; non-zero stops the scenario for good
+    // AF_SCN_* bits; non-zero stops the scenario for good.  Step k reads
+    // status[k & 1] and sets bits in status[(k + 1) & 1] (the first tile's
+    // wave carries the old bits over), so no wave of a step sees a bit that
+    // another wave of the same step set: every wave of a scenario takes the
+    // same skip decision.
+    int status[2];
 };
 
 // Ctl.status bits (the values of AF_SCN_NONFINITE / AF_SCN_STALLED).
@@ -54,12 +59,15 @@ struct ScnDev {
     int nstrips, tile_rows;
     int tile_base;                // first global tile of this scenario
     int ntiles;
-    void* z;                      // T[ny*P]; NODATA cells hold `nodata_t`
+    void* z;                      // Zt<T>[ny*P]: FP64 elevations (NODATA cells hold
+                                  // nodata_t), or FP32-path fixed point (INT_MIN)
     void* H[2];                   // T[ny*P]
     void* Qx[2];                  // T[ny*P], face f of row j at j*P + f, f in [0, nx]
     void* Qy[2];                  // T[(ny+1)*P], face row g (north of row g) at g*P + i
     double dx, dxinv, dA, fn, thresh, maxh, init_tstep, cfl, g, theta, sim_time, mass_int;
-    double nodata_t;              // sentinel compared against z (as T)
+    double nodata_t;              // the FP64 terrain's NODATA value
+    double zscale;                // metres per fixed-point unit of the FP32 terrain
+    double zbase;                 // elevation of fixed-point zero
     int west_free, east_free, north_free, south_free;
     // point sources (QFIX/QVAR) and overrides (FREE/HFIX/HVAR), sorted per
     // tile by (row, column, file order); *_begin has ntiles+1 entries.  An
@@ -159,12 +167,31 @@ __device__ __forceinline__ T load_one(gptr<const T> base, long long P, int r, in
     return base[(long long)rr * P + col];
 }
 
+// Terrain element of a state precision: FP64 keeps the elevations as given;
+// FP32 keeps them as 32-bit fixed point above a base (integer units of a
+// power-of-two step, NODATA = INT_MIN), so that every face's terrain step is
+// an exact integer difference (terrain_step).
+template <typename T>
+struct TerrainOf {
+    using type = T;
+};
+template <>
+struct TerrainOf<float> {
+    using type = int;
+};
+template <typename T>
+using Zt = typename TerrainOf<T>::type;
+
 // One row of the sliding window: V columns of H, z, Qx and Qy per lane, plus
 // the strip-edge value of each (lane 0: column i0-1; lane 63: column i0+W).
 template <typename T, int V>
 struct RowData {
-    T h[V], z[V], qx[V], qy[V];
-    T hH, zH, qyH, qxH;
+    T h[V];
+    Zt<T> z[V];
+    T qx[V], qy[V];
+    T hH;
+    Zt<T> zH;
+    T qyH, qxH;
 };
 
 // Results of one row before they are stored: H of the row, the west x-face
@@ -179,13 +206,16 @@ struct RowOut {
 // Per-tile constants of one step.
 template <typename T>
 struct TileCtx {
-    gptr<const T> z, Ho, Qxo, Qyo;
+    gptr<const Zt<T>> z;
+    gptr<const T> Ho, Qxo, Qyo;
     gptr<T> Hn, Qxn, Qyn;
     long long P;
     int nx, ny, i0, j0, j1, c0, rc, rf, lane;
     int hcol_c, hcol_f;
     bool hcell, hface;
-    T nod, sc, dxT, gT, dtdA, rainT;
+    Zt<T> nod;       // the terrain's NODATA value
+    T znod;          // FP32: the NODATA value in metres above the terrain base
+    T sc, dxT, gT, dtdA, rainT;
     bool rain_on, report_due;
     double t, dt, dA;
     FluxConstsF64 kd;
@@ -224,23 +254,37 @@ struct Stepper {
 
     // Interior face flux from stored discharges: Q0 of the face and the four
     // Q of its transverse average (kernels.cpp:79, 129).
-    static __device__ __forceinline__ T flux(T z0, T z1, T h0, T h1, T Q0, T a, T b, T c, T d,
-                                             T nod, T sc, const FluxConstsF64& kd,
+    static __device__ __forceinline__ T flux(Zt<T> z0, Zt<T> z1, T h0, T h1, T Q0, T a, T b, T c,
+                                             T d, Zt<T> nod, T sc, const FluxConstsF64& kd,
                                              const FluxConstsF32& kf) {
         if constexpr (F64) {
             if (z0 == nod || z1 == nod) return T(0);
             return face_flux_ref_t1_q(z0, z1, h0, h1, Q0, a, b, c, d, sc, kd);
         } else {
             // no branch (see face_flux_f32)
-            const T q = face_flux_f32_q(z0, z1, h0, h1, Q0, a, b, c, d, kf);
+            const T q = face_flux_f32_q(terrain_step(z0, z1, kf.zsc), h0, h1, Q0, a, b, c, d, kf);
             return (z0 == nod || z1 == nod) ? T(0) : q;
+        }
+    }
+
+    // West edge face of row c (kernels.cpp:155-179)
+    static __device__ __forceinline__ T west_edge(const TileCtx<T>& x, const RowData<T, V>& c) {
+        if constexpr (F64)
+            return edge_west<T>(c.z[0], c.h[0], c.z[1], c.h[1], c.z[0] == x.nod, x.dxT, x.gT);
+        else {
+            // the inner neighbour enters without a NODATA test (kernels.cpp:162-164):
+            // a NODATA neighbour stands at the NODATA value itself
+            const float dz10 = (c.z[1] == x.nod)
+                                   ? x.znod - (float)c.z[0] * x.kf.zsc
+                                   : terrain_step(c.z[1], c.z[0], x.kf.zsc);
+            return edge_west_f32(c.h[0], c.h[1], dz10, c.z[0] == x.nod, x.dxT, x.gT);
         }
     }
 
     // Row r of H, z, Qx (cell rows, r < rc) and Qy (face rows, r < rf).
     static __device__ __forceinline__ void load(RowData<T, V>& d, const TileCtx<T>& x, int r) {
         load_row<T, V>(d.h, x.Ho, x.P, r, x.rc, x.c0);
-        load_row<T, V>(d.z, x.z, x.P, r, x.rc, x.c0);
+        load_row<Zt<T>, V>(d.z, x.z, x.P, r, x.rc, x.c0);
         load_row<T, V>(d.qx, x.Qxo, x.P, r, x.rc, x.c0);
         load_row<T, V>(d.qy, x.Qyo, x.P, r, x.rf, x.c0);
         d.hH = load_one(x.Ho, x.P, r, x.rc, x.hcol_c);
@@ -257,11 +301,12 @@ struct Stepper {
                                                    const RowData<T, V>& s, int j,
                                                    RowOut<T, V>& o) {
         const int lane = x.lane, c0 = x.c0, nx = x.nx;
-        const T nod = x.nod, sc = x.sc;
+        const Zt<T> nod = x.nod;
+        const T sc = x.sc;
 
         // neighbours across lanes for rows j and j+1
         const T h_prev = lane_prev(c.h[V - 1], c.hH, lane);
-        const T z_prev = lane_prev(c.z[V - 1], c.zH, lane);
+        const Zt<T> z_prev = lane_prev(c.z[V - 1], c.zH, lane);
         const T qyc_prev = lane_prev(c.qy[V - 1], c.qyH, lane);
         const T qys_prev = lane_prev(s.qy[V - 1], s.qyH, lane);
         const T qxc_next = lane_next(c.qx[0], c.qxH, lane);
@@ -272,7 +317,7 @@ struct Stepper {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             const int f = c0 + v;
-            T zl = (v == 0) ? z_prev : c.z[v - 1];
+            Zt<T> zl = (v == 0) ? z_prev : c.z[v - 1];
             T hl = (v == 0) ? h_prev : c.h[v - 1];
             T ql0 = (v == 0) ? qyc_prev : c.qy[v - 1];
             T ql1 = (v == 0) ? qys_prev : s.qy[v - 1];
@@ -281,9 +326,7 @@ struct Stepper {
                 q = flux(zl, c.z[v], hl, c.h[v], c.qx[v], ql0, c.qy[v], ql1, s.qy[v], nod, sc, x.kd,
                          x.kf);
             } else if (f == 0) {
-                q = S.west_free
-                        ? edge_west<T>(c.z[0], c.h[0], c.z[1], c.h[1], c.z[0] == nod, x.dxT, x.gT)
-                        : T(0);
+                q = S.west_free ? west_edge(x, c) : T(0);
             } else if (f == nx) {
                 q = S.east_free ? edge_out_pos<T>(hl, zl == nod, x.dxT, x.gT) : T(0);
             } else {
@@ -566,7 +609,7 @@ struct Stepper {
         const double t = ctl->t[cur];
         const double next_mass = ctl->next_mass[cur];
         const double hmax_in = fkey_inv_d(ctl->hmax[hs_cur]);
-        const int status = ctl->status;
+        const int status = ctl->status[cur];
         const double dt = cfl_dt(hmax_in, S.thresh, S.cfl, S.dx, S.g, S.init_tstep);
         // a step that cannot move the clock (dt lost against t) would repeat
         // forever at the same t: every wave sees the same t and dt, so all of
@@ -578,7 +621,7 @@ struct Stepper {
 
         if (td.first) bookkeeping(S, cur, nxt, hs_cur, hs_nxt, hs_rst, s_in, t, dt, next_mass,
                                   skip, report_due, lane);
-        if (td.first && stalled && lane == 0) atomicOr(&S.ctl->status, kStatusStalled);
+        if (td.first && stalled && lane == 0) atomicOr(&S.ctl->status[nxt], kStatusStalled);
         if (skip) return;
 
         TileCtx<T> x;
@@ -593,8 +636,10 @@ struct Stepper {
         x.lane = lane;
         x.lrc = lrc;
         x.lval = lval;
-        x.nod = (T)S.nodata_t;
-        x.z = glob<const T>(S.z);
+        if constexpr (F64) x.nod = S.nodata_t;
+        else x.nod = INT_MIN;
+        x.znod = (T)(S.nodata_t - S.zbase);
+        x.z = glob<const Zt<T>>(S.z);
         x.Ho = glob<const T>(S.H[s_in]);
         x.Hn = glob<T>(S.H[s_out]);
         x.Qxo = glob<const T>(S.Qx[s_in]);
@@ -607,7 +652,7 @@ struct Stepper {
         x.kd = FluxConstsF64{S.dx, S.fn, S.thresh, S.maxh, S.g * dt, S.theta};
         x.kf = FluxConstsF32{(float)S.dx, (float)S.thresh, (float)S.maxh,
                              (float)(S.g * dt / S.dx), (float)(S.g * dt * S.fn * S.fn),
-                             (float)S.dxinv, (float)(0.25 * S.dxinv)};
+                             (float)S.dxinv, (float)(0.25 * S.dxinv), (float)S.zscale};
         x.t = t;
         x.dt = dt;
         x.dA = S.dA;
@@ -628,8 +673,11 @@ struct Stepper {
         const int hcol = (lane == 0) ? x.i0 - 1 : x.i0 + W;
         x.hcell = !EDGE || ((lane == 0) ? (x.i0 > 0) : (x.i0 + W < x.nx));  // a cell column
         x.hface = (lane != 0) && (!EDGE || x.i0 + W <= x.nx);              // an x-face column
-        x.hcol_c = x.hcell ? hcol : 0;
-        x.hcol_f = x.hface ? hcol : 0;
+        // a halo column outside the grid is loaded from the wave's own first
+        // column instead: that line is fetched anyway, and the load keeps the
+        // wait counts the same on every path
+        x.hcol_c = x.hcell ? hcol : x.i0;
+        x.hcol_f = x.hface ? hcol : x.i0;
         // rows are needed up to j1 (the south neighbour of the tile's last row)
         x.rc = min(x.ny, x.j1 + 1);
         x.rf = min(x.ny + 1, x.j1 + 1);
@@ -655,9 +703,10 @@ struct Stepper {
                 *(gptr<Vec<T, V>>)(x.Qyn + x.c0) = o;
             }
         } else {
-            T ha[V], za[V], qxa[V];
+            T ha[V], qxa[V];
+            Zt<T> za[V];
             load_row<T, V>(ha, x.Ho, x.P, j0 - 1, x.ny, x.c0);
-            load_row<T, V>(za, x.z, x.P, j0 - 1, x.ny, x.c0);
+            load_row<Zt<T>, V>(za, x.z, x.P, j0 - 1, x.ny, x.c0);
             load_row<T, V>(qxa, x.Qxo, x.P, j0 - 1, x.ny, x.c0);
             T qxH_a = load_one(x.Qxo, x.P, j0 - 1, x.ny, x.hcol_f);
             T qxa_n = lane_next(qxa[0], qxH_a, lane);
@@ -716,7 +765,7 @@ struct Stepper {
         // a non-finite depth stops the scenario from the next step on (the
         // bit is sticky; steps already under way finish or skip, and the
         // scenario's results are not used)
-        if (__any(!st.finite) && lane == 0) atomicOr(&S.ctl->status, kStatusNonfinite);
+        if (__any(!st.finite) && lane == 0) atomicOr(&S.ctl->status[nxt], kStatusNonfinite);
 
         // wavefront max, one atomic per wavefront
         T hm = st.hmax;
@@ -762,6 +811,9 @@ struct Stepper {
         gptr<Ctl> ctl = glob<Ctl>(S.ctl);
         finalize_report(S, lane);
         if (lane == 0) {
+            // the status bits step k+1 reads: those of step k, and any that
+            // the waves of step k set (atomically) into the same word
+            atomicOr(&S.ctl->status[nxt], ctl->status[cur]);
             ctl->hmax[hs_rst] = key_neg_max();  // the slot step k+1 max-reduces into
             if (!skip) {
                 const double t_new = t + dt;  // main.cpp:287
