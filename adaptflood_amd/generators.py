@@ -186,6 +186,23 @@ def ensemble_member(k, seed_base=4000, dx=5.0, sim_time=3600.0, lo=128, hi=2048)
                     name=f"ensemble {k} {nx}x{ny}")
 
 
+def ensemble_shape(k, seed_base=4000, lo=128, hi=2048):
+    """(nx, ny) of ensemble_member(k) from its first draws alone, without
+    building the terrain (the same generator, stopped after the shape)."""
+    rng = np.random.default_rng(seed_base + k)
+    L = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+    a = rng.uniform(1.0, 4.0)
+    S = max(16, int(round(L / a)))
+    return (L, S) if rng.random() < 0.5 else (S, L)
+
+
+def smallest_members(count, n=512, seed_base=4000):
+    """Indices of the `count` ensemble members with the fewest cells."""
+    cells = [(ensemble_shape(k, seed_base)[0] * ensemble_shape(k, seed_base)[1], k)
+             for k in range(n)]
+    return [k for _, k in sorted(cells)[:count]]
+
+
 def ensemble(n=512, seed_base=4000, sim_time=3600.0, lo=128, hi=2048):
     return [ensemble_member(k, seed_base, sim_time=sim_time, lo=lo, hi=hi) for k in range(n)]
 

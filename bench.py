@@ -13,8 +13,9 @@ Other workloads (--workload): "ensemble" (512 ragged scenarios in one launch
 per step against sequential single-scenario runs), "served" (the served
 ensemble with its sequential baselines), "accuracy" (FP32/FP64 errors against
 FP64 references), "divergence" (lockstep drift at equal step counts), "shard"
-(one GPU's row slab), "driver" (the rebuilt reference program end to end) and
-"lone" (step time of small and mid-size domains run alone).
+(one GPU's row slab), "driver" (the rebuilt reference program end to end),
+"lone" (step time of small and mid-size domains run alone) and "twin" (the
+FP64 path against itself with one input nudged by one ulp, beside FP32).
 
 Algorithmic bytes per cell-update: read z, H, Qx_old, Qy_old and write H, Qx,
 Qy = 7 values = 28 B at FP32 (56 B at FP64).
@@ -468,6 +469,109 @@ def bench_divergence(args):
     print(json.dumps(out), flush=True)
 
 
+def _nudged(scn, how="source"):
+    """The same case with inputs moved by one ulp.  how="source": one input,
+    the first QFIX value, or else the peak of the first QVAR hydrograph (a
+    perturbation at one place).  how="dem": every valid terrain value (a
+    perturbation of every face at FP64's rounding level, as FP32 rounding
+    perturbs every cell)."""
+    import dataclasses
+    from adaptflood_amd._lib import AF_BC_QFIX, AF_BC_QVAR
+    if how == "dem":
+        dem = np.array(scn.dem, dtype=np.float64)
+        valid = dem != scn.nodata
+        dem[valid] = np.nextafter(dem[valid], np.inf)
+        return dataclasses.replace(scn, dem=dem), "every valid DEM value + 1 ulp"
+    pts = [dataclasses.replace(p) for p in scn.points]
+    for p in pts:
+        if p.type == AF_BC_QFIX:
+            p.value = float(np.nextafter(p.value, np.inf))
+            return dataclasses.replace(scn, points=pts), "first QFIX value + 1 ulp"
+    for p in pts:
+        if p.type == AF_BC_QVAR:
+            v = np.array(p.values, dtype=np.float64)
+            k = int(np.argmax(v))
+            v[k] = np.nextafter(v[k], np.inf)
+            p.values = v
+            return dataclasses.replace(scn, points=pts), "first QVAR peak + 1 ulp"
+    raise ValueError("no source to nudge")
+
+
+def bench_twin(args):
+    """FP64 twin: the FP64 fused path on a case and on the same case with one
+    source nudged by one ulp, beside the FP32 path, in lockstep at equal step
+    counts (then each to its end).  If the FP64 twin drifts from the FP64 run
+    as far as FP32 does, the FP32 gap is the flow amplifying any difference,
+    not an FP32 accounting error; the longest window in which the twin stays
+    within 1e-9 m is where a non-bitwise path can be held to the bars.
+    --twin-cases: comma-separated case:size:sim_time (case valley or roofline)."""
+    import torch
+    from adaptflood_amd import generators
+    from adaptflood_amd.domain import Domain, make_tuning
+    out = {"metric": "FP64 twin (one input +1 ulp) and FP32 against FP64, equal step counts",
+           "cases": []}
+    for spec in args.twin_cases.split(","):
+        name, size, sim = spec.split(":")
+        size, sim = int(size), float(sim)
+        t = time.perf_counter()
+        base = (generators.valley_case(n=size) if name == "valley"
+                else generators.roofline_case(n=size))
+        base.sim_time = sim
+        twin, how = _nudged(base, args.twin_nudge)
+        print(f"[twin] {name} {size}^2 generated in {time.perf_counter() - t:.1f}s ({how})",
+              file=sys.stderr, flush=True)
+        dA = base.dx * base.dx
+        tun = make_tuning(graph_steps=36)
+        a = Domain(base, precision="fp64", tuning=tun)
+        b = Domain(twin, precision="fp64", tuning=tun)
+        c = Domain(base, precision="fp32", tuning=tun)
+        rows, done, nxt = [], 0, 25
+        t0 = time.perf_counter()
+
+        def compare(label):
+            Ha = a.read("H")
+            Hb, Hc = b.read("H"), c.read("H")
+            ia, ib, ic = a.info(), b.info(), c.info()
+            eb, ec = _errors(Hb, Ha, dA), _errors(Hc, Ha, dA)
+            row = {"at": label, "steps": [ia["steps"], ib["steps"], ic["steps"]],
+                   "t": ia["t"],
+                   "twin_max": eb["max_abs_dH_m"], "twin_rms": eb["rms_dH_m"],
+                   "twin_mass": eb["rel_mass_err"], "twin_cells": eb["area_mismatch_cells"],
+                   "fp32_max": ec["max_abs_dH_m"], "fp32_rms": ec["rms_dH_m"],
+                   "fp32_mass": ec["rel_mass_err"], "fp32_cells": ec["area_mismatch_cells"]}
+            rows.append(row)
+            print(f"[twin] {name} {size}^2 {label} t={ia['t']:.1f}: twin max {row['twin_max']:.2e} "
+                  f"rms {row['twin_rms']:.2e} mass {row['twin_mass']:.2e} | fp32 max "
+                  f"{row['fp32_max']:.2e} rms {row['fp32_rms']:.2e} mass {row['fp32_mass']:.2e} "
+                  f"({time.perf_counter() - t0:.0f}s)", file=sys.stderr, flush=True)
+            return ia, ib, ic
+
+        while True:
+            for d in (a, b, c):
+                d.advance(nxt - done, graph=True)
+            done = nxt
+            ia, ib, ic = compare(f"step {done}")
+            if ia["finished"] or ib["finished"] or ic["finished"]:
+                break
+            nxt = done * 2 if done < args.twin_linear else done + args.twin_linear
+        # each run to its own end, then the final states
+        for d in (a, b, c):
+            while not d.info()["finished"]:
+                d.advance(2160, graph=True)
+        compare("end")
+        window = 0
+        for r in rows:
+            if r["at"].startswith("step") and r["twin_max"] <= 1e-9:
+                window = r["steps"][0]
+        out["cases"].append({"case": f"{name} {size}x{size}, {sim:.0f} s", "nudge": how,
+                             "twin_within_1e-9_m_through_step": window, "rows": rows})
+        for d in (a, b, c):
+            d.close()
+        del a, b, c
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
 def _asc_body(path):
     """The grid lines of an ESRI-ASCII file (the six header lines dropped)."""
     with open(path, "rb") as f:
@@ -656,9 +760,15 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
                     choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard",
-                             "driver", "lone"),
+                             "driver", "lone", "twin"),
                     default="roofline")
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
+    ap.add_argument("--twin-cases", default="roofline:2048:3600,valley:4096:7200,"
+                    "roofline:16384:3600", help="twin: case:size:sim_time list")
+    ap.add_argument("--twin-nudge", choices=("source", "dem"), default="dem",
+                    help="twin: one source value +1 ulp, or every DEM value +1 ulp")
+    ap.add_argument("--twin-linear", type=int, default=3200,
+                    help="twin: checkpoint spacing once the doubling schedule reaches it")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
     ap.add_argument("--acc-time", type=float, default=600.0,
                     help="accuracy: simulated seconds of the valley comparison with the oracle")
@@ -706,6 +816,8 @@ def main():
         return bench_driver(args)
     if args.workload == "lone":
         return bench_lone(args)
+    if args.workload == "twin":
+        return bench_twin(args)
     served_reqs = None
     if not args.no_served:
         # generated in worker processes before this process touches the GPU
