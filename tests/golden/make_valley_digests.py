@@ -1,0 +1,140 @@
+"""Writes the digests of the FP64 oracle's full run of the 4096 x 4096 valley
+workload (fractal terrain with a carved valley at 5 m, 2000 QVAR hydrographs,
+20 mm/h rain over 0-1800 s, HFIX 2 m west edge, FREE N/E/S, 7200 s).
+
+The full run is about 76,000 steps, roughly an hour of the CPU oracle, so
+the fixture keeps SHA-256 digests of the state instead of the state itself:
+H (ny*nx), Qx and Qy ((ny+1)*(nx+1), pitch nx+1) as little-endian float64,
+at every CHECK steps and at the end, the digest of the whole dt log, the step
+count, the final t and every mass report (t, dt, Vol, Area).  A GPU test
+advances the FP64 fused path by the same step counts and compares digests;
+the reports are compared as values (the volume sum's order differs).
+
+Run from the repository root:
+    python tests/golden/make_valley_digests.py [--threads N] [--checkpoint DIR]
+With --checkpoint, the oracle state is saved to DIR after every CHECK steps
+and a later run with the same DIR resumes from it.  With --emit, the fixture
+is written from the checkpoints in DIR so far (no stepping): its "final" is
+null until the run has ended, and "dt_sha256" covers the first "dt_steps"
+entries of the dt log.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from adaptflood_amd import generators  # noqa: E402
+from oracle import oracle  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "valley4096_fp64.json")
+CHECK = 5000
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def save_state(d, r, checkpoints):
+    os.makedirs(d, exist_ok=True)
+    tmp = os.path.join(d, "state.tmp.npz")
+    np.savez(tmp, H=r.H, Qx=r.Qx, Qy=r.Qy, Qxold=r.Qxold, Qyold=r.Qyold,
+             scalars=np.array([r.t, r.next_mass, r.steps, r.step_seconds]),
+             dts=np.array(r.dts), reports=np.array(r.reports).reshape(-1, 4))
+    os.replace(tmp, os.path.join(d, "state.npz"))
+    with open(os.path.join(d, "checkpoints.json"), "w") as f:
+        json.dump(checkpoints, f)
+
+
+def load_state(d, r):
+    p = os.path.join(d, "state.npz")
+    if not os.path.exists(p):
+        return []
+    with np.load(p, allow_pickle=False) as z:
+        for k in ("H", "Qx", "Qy", "Qxold", "Qyold"):
+            getattr(r, k)[:] = z[k]
+        t, nm, steps, secs = z["scalars"].tolist()
+        r.t, r.next_mass, r.steps, r.step_seconds = t, nm, int(steps), secs
+        r.dts = z["dts"].tolist()
+        r.reports = z["reports"].tolist()
+    with open(os.path.join(d, "checkpoints.json")) as f:
+        return json.load(f)
+
+
+def emit(d):
+    """The fixture from the checkpoints saved so far."""
+    with open(os.path.join(d, "checkpoints.json")) as f:
+        checkpoints = json.load(f)
+    with np.load(os.path.join(d, "state.npz"), allow_pickle=False) as z:
+        dts = z["dts"]
+        reports = z["reports"]
+        steps = int(z["scalars"][2])
+    assert checkpoints and checkpoints[-1]["steps"] == steps == len(dts)
+    scn = generators.valley_case(n=4096)
+    write_fixture(scn, checkpoints, None, dts, reports, None)
+
+
+def write_fixture(scn, checkpoints, final, dts, reports, seconds):
+    out = {
+        "workload": "valley 4096x4096 at 5 m, 2000 QVAR, rain 0-1800 s, HFIX west, 7200 s",
+        "dem_sha256": hashlib.sha256(np.ascontiguousarray(scn.dem, dtype=np.float64)
+                                     .tobytes()).hexdigest(),
+        "check_every": CHECK,
+        "checkpoints": checkpoints,
+        "final": final,
+        "dt_steps": int(len(dts)),
+        "dt_sha256": sha(dts),
+        "dt_first": np.asarray(dts[:8]).tolist(),
+        "dt_last": np.asarray(dts[-8:]).tolist(),
+        "reports": np.asarray(reports).tolist(),
+        "oracle_seconds": seconds,
+    }
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=1)
+    end = final or checkpoints[-1]
+    print(f"wrote {OUT}: {end['steps']} steps, t={end['t']:.6f}"
+          + ("" if final else " (partial)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--checkpoint", default=None)
+    ap.add_argument("--emit", action="store_true")
+    a = ap.parse_args()
+    if a.emit:
+        return emit(a.checkpoint)
+    if a.threads:
+        oracle.set_threads(a.threads)
+    scn = generators.valley_case(n=4096)
+    r = oracle.OracleRun(scn, dt_log_cap=1 << 18, mass_log_cap=1024)
+    checkpoints = load_state(a.checkpoint, r) if a.checkpoint else []
+    if checkpoints:
+        print(f"resumed at step {r.steps}, t={r.t:.3f}", flush=True)
+    t0 = time.time()
+    while r.t < scn.sim_time:
+        n = r.advance(CHECK)
+        st = r.state()
+        rec = {"steps": st["steps"], "t": st["t"], "H": sha(st["H"]), "Qx": sha(st["Qx"]),
+               "Qy": sha(st["Qy"]), "H_max": float(st["H"].max()),
+               "H_sum": float(st["H"].sum())}
+        print(f"{time.time() - t0:8.0f} s  steps={st['steps']} t={st['t']:.3f} "
+              f"H_max={rec['H_max']:.6f}", flush=True)
+        if n == CHECK and r.t < scn.sim_time:
+            checkpoints.append(rec)
+            if a.checkpoint:
+                save_state(a.checkpoint, r, checkpoints)
+        else:
+            final = rec
+    st = r.state()
+    write_fixture(scn, checkpoints, final, st["dt"], st["reports"], time.time() - t0)
+
+
+if __name__ == "__main__":
+    main()
