@@ -97,6 +97,16 @@ double orc_pow_10_3(double h) {
     return ldexp(hi + (lo + d), 10 * q);
 }
 
+/* The flux's pow(hflow, 10.0/3.0).  Built with -DORC_LIBM_POW (the second
+ * library of oracle/Makefile), it is the C library's pow, as in the
+ * reference's CPU build (kernels.cpp:83, 133): that build measures how far
+ * the choice of pow rounding moves a whole run from the pinned oracle. */
+#ifdef ORC_LIBM_POW
+#define ORC_POW_10_3(h) pow((h), 10.0 / 3.0)
+#else
+#define ORC_POW_10_3(h) orc_pow_10_3(h)
+#endif
+
 /* kernels.cpp:6-19 */
 void orc_update_qs(const double* Qx, const double* Qy, double* Qxold, double* Qyold,
                    int xsz, int ysz, double dx) {
@@ -144,7 +154,7 @@ double orc_face_flux(double z0, double z1, double h0, double h1, double q0, doub
     if (hflow > DepthThresh) {
         double qvect = sqrt(q0 * q0 + qt * qt);
         double numerator = (theta * q0 + 0.5 * (1.0 - theta) * (qup + qdown)) - (g * dt * hflow * Sf);
-        double denominator = 1.0 + g * dt * hflow * fn * fn * fabs(qvect) / orc_pow_10_3(hflow);
+        double denominator = 1.0 + g * dt * hflow * fn * fn * fabs(qvect) / ORC_POW_10_3(hflow);
         Q = numerator / denominator * dx;
         if (Q * dh < 0.0) {
             numerator = q0 - (g * dt * hflow * Sf);

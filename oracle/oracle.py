@@ -12,6 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
+# the same oracle with the C library's pow(hflow, 10.0/3.0) in the flux
+LIB_LIBM = os.path.join(HERE, "_build", "liboracle_libm.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -44,21 +46,22 @@ class OrcRunInfo(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build(force=False):
-    if force or not os.path.exists(LIB):
+    if force or not (os.path.exists(LIB) and os.path.exists(LIB_LIBM)):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-f",
                         os.path.join(HERE, "Makefile")], check=True)
     return LIB
 
 
-def lib():
-    global _lib
-    if _lib is None:
+def lib(libm=False):
+    """The pinned oracle, or with libm=True the build whose flux takes the C
+    library's pow (for measuring what that rounding does to a run)."""
+    if libm not in _libs:
         build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(LIB_LIBM if libm else LIB)
         V = C.c_void_p
         D = C.c_double
         I = C.c_int
@@ -85,8 +88,8 @@ def lib():
         L.orc_run.restype = I
         L.orc_num_threads.restype = I
         L.orc_set_num_threads.argtypes = [I]
-        _lib = L
-    return _lib
+        _libs[libm] = L
+    return _libs[libm]
 
 
 def _p(a):
@@ -163,8 +166,9 @@ class _Packed:
 class OracleRun:
     """State of one oracle run; step it with advance()."""
 
-    def __init__(self, s, dt_log_cap=1 << 20, mass_log_cap=4096):
+    def __init__(self, s, dt_log_cap=1 << 20, mass_log_cap=4096, libm=False):
         self.s = s
+        self.libm = libm
         self.pk = _Packed(s)
         ny, nx = s.dem.shape
         self.H = np.zeros(nx * ny)
@@ -189,7 +193,7 @@ class OracleRun:
         ml = np.zeros(4 * self._mass_cap)
         info.dt_log, info.dt_log_cap = dtl.ctypes.data_as(_dp), self._dt_cap
         info.mass_log, info.mass_log_cap = ml.ctypes.data_as(_dp), self._mass_cap
-        rc = lib().orc_run(C.byref(self.pk.case), _p(self.H), _p(self.pk.dem), _p(self.Qx),
+        rc = lib(self.libm).orc_run(C.byref(self.pk.case), _p(self.H), _p(self.pk.dem), _p(self.Qx),
                            _p(self.Qy), _p(self.Qxold), _p(self.Qyold), self.t, self.next_mass,
                            C.byref(info))
         if rc != 0:
@@ -211,8 +215,8 @@ class OracleRun:
                 "reports": np.array(self.reports).reshape(-1, 4)}
 
 
-def run(s, max_steps=None):
-    r = OracleRun(s)
+def run(s, max_steps=None, libm=False):
+    r = OracleRun(s, libm=libm)
     r.advance(max_steps)
     return r.state()
 
