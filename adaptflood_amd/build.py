@@ -3,6 +3,7 @@
 Produces, in-tree:
   adaptflood_amd/_lib/libadaptflood_amd.so   the C ABI of include/adaptflood.h
   adaptflood_amd/_lib/lisflood_amd           the command-line driver
+  adaptflood_amd/_lib/af_hbm_probe           the bandwidth probe of the step's memory pattern
 
 Every HIP translation unit is compiled with -ffp-contract=off (the FP64 path
 must evaluate the reference's operations without contracted FMAs) and with
@@ -66,6 +67,10 @@ def build(verbose=False):
         exe = os.path.join(OUT, "lisflood_amd")
         _run(["g++", drv_obj, "-o", exe, "-L", OUT, "-ladaptflood_amd",
               "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    # the bandwidth probe of the step's memory pattern (bench.py --workload ceiling)
+    probe_src = os.path.join(CSRC, "hbm_probe.hip")
+    if os.path.exists(probe_src):
+        _run([HIPCC, *HIP_FLAGS, probe_src, "-o", os.path.join(OUT, "af_hbm_probe")])
     if verbose:
         print("built", lib)
     return lib
