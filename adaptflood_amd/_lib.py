@@ -122,9 +122,15 @@ SIGNATURES = [
                                     C.POINTER(C.c_size_t), _ip]),
     ("af_pool_create", _I, [_I, C.POINTER(Tuning), _I, _I, _V, C.c_size_t, C.POINTER(_V)]),
     ("af_pool_admit", _I, [_V, C.POINTER(ScenarioDesc), _V, _ip]),
+    ("af_pool_stage", _I, [C.POINTER(ScenarioDesc), _I, C.POINTER(_V)]),
+    ("af_pool_admit_staged", _I, [_V, _V, _V, _ip]),
+    ("af_staged_free", None, [_V]),
     ("af_pool_release", _I, [_V, _I32, _V]),
     ("af_pool_poll", _I, [_V, _ip, _I32, _ip, _ip, _V]),
     ("af_pool_progress", _I, [_V, _ip, _dp, C.POINTER(_I64), _ip, _ip, _I32, _ip, _V]),
+    ("af_pool_set_lane", _I, [_V, _ip, _I32, _V]),
+    ("af_pool_advance_lane", _I, [_V, _I64, _I, _V]),
+    ("af_pool_lane_ms", _I, [_V, C.POINTER(C.c_float)]),
     ("af_pool_stats", _I, [_V, _ip, _ip, C.POINTER(C.c_size_t), C.POINTER(_I64)]),
     ("af_par_defaults", None, [C.POINTER(Par)]),
     ("af_read_par", _I, [C.c_char_p, C.POINTER(Par)]),

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -56,10 +58,95 @@ struct HostScn {
     bool active = false;
     size_t base = 0, bytes = 0;  // heap extent of this scenario in the arena
     int base_parity = 0;         // state buffer holding the initial state
+    bool in_lane = false;        // stepped by the lane (af_pool_set_lane), not the main table
+};
+
+// Host staging memory: pinned (page-locked) blocks, so that the copies to the
+// device are plain DMA transfers that do not pass through the runtime's
+// pageable staging.  Blocks come in power-of-two sizes from 64 KiB and go
+// back to a process-wide cache when released, never to the system:
+// pinning is slow, and hipHostFree can synchronise the device.  The cache
+// therefore holds the peak of staging memory in use at once.  Where pinned
+// memory cannot be had, a block falls back to ordinary memory.
+class HostBuf {
+  public:
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void* data() const { return p_; }
+    bool reserve(size_t bytes) {
+        release();
+        size_t cls = size_t(1) << 16;
+        while (cls < bytes) cls <<= 1;
+        {
+            std::lock_guard<std::mutex> g(mu());
+            auto it = cache().find(cls);
+            if (it != cache().end()) {
+                p_ = it->second;
+                cache().erase(it);
+                cls_ = cls;
+                pinned_ = true;
+                return true;
+            }
+        }
+        if (hipHostMalloc(&p_, cls, hipHostMallocPortable) == hipSuccess) {
+            pinned_ = true;
+        } else {
+            (void)hipGetLastError();
+            p_ = malloc(cls);
+            pinned_ = false;
+            if (!p_) return false;
+        }
+        cls_ = cls;
+        return true;
+    }
+    void release() {
+        if (!p_) return;
+        if (pinned_) {
+            std::lock_guard<std::mutex> g(mu());
+            cache().emplace(cls_, p_);
+        } else {
+            free(p_);
+        }
+        p_ = nullptr;
+    }
+
+  private:
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::multimap<size_t, void*>& cache() {
+        static std::multimap<size_t, void*> c;
+        return c;
+    }
+    void* p_ = nullptr;
+    size_t cls_ = 0;
+    bool pinned_ = false;
+};
+
+// A scenario's terrain and initial depths in the device format, converted on
+// the host before admission (stage_inputs): the part of an admission that
+// needs neither the pool nor the GPU, so it can run while the pool steps.
+struct Staged {
+    int precision = AF_FP32;
+    int nx = 0, ny = 0;
+    long long P = 0;
+    HostBuf z, h;   // P*ny elements each: FP64 elevations and depths, or the
+                    // FP32 path's fixed-point terrain (int) and float depths
+    double zbase = 0.0, zscale = 1.0, hmax = -DBL_MAX;
 };
 
 }  // namespace
 }  // namespace af
+
+// A scenario staged by af_pool_stage: a copy of its descriptor (whose arrays
+// the caller keeps) and its converted terrain and depths.
+struct af_staged {
+    af_scenario_desc desc{};
+    af::Staged s;
+};
 
 struct af_domain {
     int precision = AF_FP32;
@@ -77,6 +164,7 @@ struct af_domain {
     std::vector<af::HostScn> scn;  // max_scn slots
     std::vector<std::pair<size_t, size_t>> free_list;  // (offset, size), sorted
     size_t o_scns = 0, o_tiles = 0, o_volbuf = 0, o_ctlptr = 0, o_ctlbuf = 0, heap_lo = 0;
+    size_t o_lane = 0;
     af::ScnDev* d_scns = nullptr;
     af::TileDesc* d_tiles = nullptr;
     double* d_volbuf = nullptr;
@@ -84,10 +172,26 @@ struct af_domain {
     af::Ctl* d_ctlbuf = nullptr;      // their gathered copies
     std::vector<int> active;          // active slots, ascending (as of the last tile rebuild)
     int ntiles = 0;
+    bool dirty = false;               // admissions or releases since the last tile rebuild
     long long k = 0;  // index of the next step to launch
     double last_tsum = -1.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    // the lane: a second tile table holding every tile of some scenarios
+    // (af_pool_set_lane), which the main table then leaves out;
+    // af_pool_advance_lane steps them on a stream of its own, concurrently
+    // with the main table's launches, until a call that joins it
+    std::vector<int> lane_slots;
+    af::TileDesc* d_lane = nullptr;
+    int nlane = 0;
+    hipGraph_t lgraph = nullptr;
+    hipGraphExec_t lgexec = nullptr;
+    hipStream_t lane_st = nullptr;
+    hipEvent_t ev_fork = nullptr;     // the caller's stream when the lane work was enqueued
+    hipEvent_t ev_t0 = nullptr;       // the lane work's start and end (timed)
+    hipEvent_t ev_join = nullptr;
+    bool lane_pending = false;        // lane work not yet joined into the caller's stream
+    bool lane_timed = false;          // ev_t0 / ev_join hold a lane burst
 };
 
 namespace af {
@@ -180,22 +284,27 @@ void rebase(HostScn& h, size_t b) {
     h.base = b;
 }
 
-void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
-    h.nx = d->nx;
-    h.ny = d->ny;
-    // Row pitch: wide rows start on an odd multiple of 1 KiB, which spreads
-    // the rows a strip marches down over the HBM channels (a 256-byte-aligned
-    // pitch of nx+1 elements measured 7% slower on the 4-read/3-write row
-    // march at 16384 columns); narrow rows keep 256-byte alignment.
-    const size_t es = elem_size(D->precision);
-    size_t row = ((size_t)d->nx + 1) * es;
+// Row pitch in elements: wide rows start on an odd multiple of 1 KiB, which
+// spreads the rows a strip marches down over the HBM channels (a
+// 256-byte-aligned pitch of nx+1 elements measured 7% slower on the
+// 4-read/3-write row march at 16384 columns); narrow rows keep 256-byte
+// alignment.
+long long row_pitch(int nx, int precision) {
+    const size_t es = elem_size(precision);
+    size_t row = ((size_t)nx + 1) * es;
     if (row >= 8192) {
         row = align_up(row, 1024);
         if (((row >> 10) & 1) == 0) row += 1024;
     } else {
         row = align_up(row, 256);
     }
-    h.P = (long long)(row / es);
+    return (long long)(row / es);
+}
+
+void plan_geometry(HostScn& h, const af_scenario_desc* d, const af_domain* D) {
+    h.nx = d->nx;
+    h.ny = d->ny;
+    h.P = row_pitch(d->nx, D->precision);
     h.nstrips = (d->nx + D->W - 1) / D->W;
     // Tile rows: the tuning's value, else the default height, halved while
     // the scenario alone would give fewer than kMinTiles wavefronts, down to
@@ -355,6 +464,7 @@ size_t plan_tables(af_domain* D, int max_scn, int max_tiles) {
     Layout L;
     D->o_scns = L.take(sizeof(ScnDev) * (size_t)max_scn);
     D->o_tiles = L.take(sizeof(TileDesc) * (size_t)max_tiles);
+    D->o_lane = L.take(sizeof(TileDesc) * (size_t)max_tiles);
     D->o_volbuf = L.take(sizeof(double) * 3 * kVolBlocks);
     D->o_ctlptr = L.take(sizeof(Ctl*) * (size_t)max_scn);
     D->o_ctlbuf = L.take(sizeof(Ctl) * (size_t)max_scn);
@@ -438,15 +548,14 @@ void terrain_scale(const af_scenario_desc* d, double& zbase, double& zscale) {
 // value itself) or initial depth: such a cell would turn its neighbours'
 // fluxes to NaN on the first step.
 template <typename T, typename Z>
-int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<Z>& z,
-                   std::vector<T>& H, double& zbase, double& zscale, double& hmax) {
-    const long long P = h.P;
+int convert_inputs(const af_scenario_desc* d, long long P, Z* z, T* H, double& zbase,
+                   double& zscale, double& hmax) {
     constexpr bool f64 = sizeof(T) == 8;
     zbase = 0.0;
     zscale = 1.0;
     if (!f64) terrain_scale(d, zbase, zscale);
-    z.assign((size_t)P * d->ny, Z(0));
-    H.assign((size_t)P * d->ny, T(0));
+    memset(z, 0, sizeof(Z) * (size_t)P * d->ny);
+    memset(H, 0, sizeof(T) * (size_t)P * d->ny);
     hmax = -DBL_MAX;
     for (int j = 0; j < d->ny; j++) {
         for (int i = 0; i < d->nx; i++) {
@@ -470,9 +579,57 @@ int convert_inputs(const af_scenario_desc* d, const HostScn& h, std::vector<Z>& 
     return AF_OK;
 }
 
+int stage_inputs(const af_scenario_desc* d, int precision, Staged& s) {
+    int rc = validate(d);
+    if (rc) return rc;
+    AF_REQUIRE(precision == AF_FP32 || precision == AF_FP64, AF_E_INVALID, "precision %d",
+               precision);
+    s.precision = precision;
+    s.nx = d->nx;
+    s.ny = d->ny;
+    s.P = row_pitch(d->nx, precision);
+    const size_t bytes = (size_t)s.P * d->ny * elem_size(precision);
+    AF_REQUIRE(s.z.reserve(bytes) && s.h.reserve(bytes), AF_E_CAPACITY,
+               "stage: no host memory for %zu bytes", 2 * bytes);
+    if (precision == AF_FP64)
+        return convert_inputs<double, double>(d, s.P, (double*)s.z.data(), (double*)s.h.data(),
+                                              s.zbase, s.zscale, s.hmax);
+    return convert_inputs<float, int>(d, s.P, (int*)s.z.data(), (float*)s.h.data(), s.zbase,
+                                      s.zscale, s.hmax);
+}
+
+// Zeroes `words` 4-byte words at p (16-byte aligned): 16-byte stores for the
+// body, single words for the tail.  (hipMemsetAsync of a few MiB measured
+// about 1 ms to enqueue on the host, longer than the fill itself.)
+__global__ void __launch_bounds__(256) k_zero(uint4* __restrict__ p, long long words) {
+    const long long n4 = words >> 2;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n4; i += stride)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+    const long long t = (n4 << 2) + blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t < words) reinterpret_cast<unsigned*>(p)[t] = 0u;
+}
+
+int zero_fill(void* p, size_t bytes, hipStream_t st) {
+    AF_REQUIRE(((uintptr_t)p & 15) == 0 && bytes % 4 == 0, AF_E_INVALID,
+               "zero_fill: unaligned range");
+    const long long words = (long long)(bytes / 4);
+    if (words == 0) return AF_OK;
+    const long long blocks = std::min<long long>((words / 4 + 255) / 256 + 1, 4096);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st,
+                       reinterpret_cast<uint4*>(p), words);
+    AF_HIP(hipGetLastError());
+    return AF_OK;
+}
+
 int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t st,
-                    int parity, long long kmod) {
+                    int parity, long long kmod, const Staged& pre) {
     HostScn& h = D->scn[s];
+    AF_REQUIRE(pre.precision == D->precision && pre.nx == d->nx && pre.ny == d->ny &&
+                   pre.P == h.P,
+               AF_E_INVALID, "staged scenario (%d x %d, %s) does not match the pool (%s)",
+               pre.nx, pre.ny, pre.precision == AF_FP64 ? "fp64" : "fp32",
+               D->precision == AF_FP64 ? "fp64" : "fp32");
     plan_points(h, d, D);
     char* A = D->arena;
     // device pointers
@@ -535,56 +692,47 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     v.reports = reinterpret_cast<double*>(A + h.o_reports);
     v.report_cap = D->report_cap;
 
-    // state: zero everything, then z and H0 into buffer `parity`
+    // state: the terrain and the initial depths (buffer `parity`) from the
+    // staged inputs, every other state buffer zero (one fill over the
+    // contiguous H, Qx, Qy extents of both buffers, before the depth copy)
     const size_t es = elem_size(D->precision);
     const size_t cells = (size_t)h.P * h.ny, faces_y = (size_t)h.P * (h.ny + 1);
-    AF_HIP(hipMemsetAsync(A + h.o_z, 0, cells * es, st));
-    for (int b = 0; b < 2; b++) {
-        AF_HIP(hipMemsetAsync(A + h.o_H[b], 0, cells * es, st));
-        AF_HIP(hipMemsetAsync(A + h.o_Qx[b], 0, cells * es, st));
-        AF_HIP(hipMemsetAsync(A + h.o_Qy[b], 0, faces_y * es, st));
-    }
-    double hmax = -DBL_MAX;
-    if (D->precision == AF_FP64) {
-        std::vector<double> z, H;
-        int crc = convert_inputs<double, double>(d, h, z, H, h.zbase, h.zscale, hmax);
-        if (crc) return crc;
-        AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
-        AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
-        AF_HIP(hipStreamSynchronize(st));
-    } else {
-        std::vector<float> H;
-        std::vector<int> z;
-        int crc = convert_inputs<float, int>(d, h, z, H, h.zbase, h.zscale, hmax);
-        if (crc) return crc;
-        AF_HIP(hipMemcpyAsync(A + h.o_z, z.data(), cells * es, hipMemcpyHostToDevice, st));
-        AF_HIP(hipMemcpyAsync(A + h.o_H[parity], H.data(), cells * es, hipMemcpyHostToDevice, st));
-        AF_HIP(hipStreamSynchronize(st));
-    }
+    AF_REQUIRE(h.o_H[0] < h.o_Qx[0] && h.o_Qy[0] < h.o_H[1] && h.o_Qx[1] < h.o_Qy[1],
+               AF_E_INVALID, "state layout out of order");
+    int zrc = zero_fill(A + h.o_H[0], h.o_Qy[1] + faces_y * es - h.o_H[0], st);
+    if (zrc) return zrc;
+    const double hmax = pre.hmax;
+    h.zbase = pre.zbase;
+    h.zscale = pre.zscale;
+    AF_HIP(hipMemcpyAsync(A + h.o_z, pre.z.data(), cells * es, hipMemcpyHostToDevice, st));
+    AF_HIP(hipMemcpyAsync(A + h.o_H[parity], pre.h.data(), cells * es, hipMemcpyHostToDevice, st));
     v.zscale = h.zscale;
     v.zbase = h.zbase;
-    auto up = [&](size_t off, const void* src, size_t bytes) -> int {
-        if (bytes == 0) return AF_OK;
-        AF_HIP(hipMemcpyAsync(A + off, src, bytes, hipMemcpyHostToDevice, st));
-        return AF_OK;
+
+    // point lists, series, rain and the control block: consecutive extents of
+    // the layout (src_begin first, the control block last), assembled on the
+    // host and uploaded with one copy
+    const size_t aux_lo = h.o_src_begin, aux_hi = h.o_ctl + sizeof(Ctl);
+    std::vector<char> aux(aux_hi - aux_lo, 0);
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes) memcpy(aux.data() + (off - aux_lo), src, bytes);
     };
-    int rc = AF_OK;
-    if ((rc = up(h.o_src_begin, h.src_begin.data(), h.src_begin.size() * 4))) return rc;
-    if ((rc = up(h.o_src_j, h.src_j.data(), h.src_j.size() * 4))) return rc;
-    if ((rc = up(h.o_src_i, h.src_i.data(), h.src_i.size() * 4))) return rc;
-    if ((rc = up(h.o_src_val, h.src_val.data(), h.src_val.size() * 8))) return rc;
-    if ((rc = up(h.o_src_off, h.src_off.data(), h.src_off.size() * 4))) return rc;
-    if ((rc = up(h.o_src_len, h.src_len.data(), h.src_len.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_begin, h.ovr_begin.data(), h.ovr_begin.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_j, h.ovr_j.data(), h.ovr_j.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_i, h.ovr_i.data(), h.ovr_i.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_val, h.ovr_val.data(), h.ovr_val.size() * 8))) return rc;
-    if ((rc = up(h.o_ovr_off, h.ovr_off.data(), h.ovr_off.size() * 4))) return rc;
-    if ((rc = up(h.o_ovr_len, h.ovr_len.data(), h.ovr_len.size() * 4))) return rc;
-    if ((rc = up(h.o_series_t, d->series_t, (size_t)h.n_series * 8))) return rc;
-    if ((rc = up(h.o_series_v, d->series_v, (size_t)h.n_series * 8))) return rc;
-    if ((rc = up(h.o_rain_t, d->rain_t, (size_t)h.n_rain * 8))) return rc;
-    if ((rc = up(h.o_rain_r, d->rain_r, (size_t)h.n_rain * 8))) return rc;
+    put(h.o_src_begin, h.src_begin.data(), h.src_begin.size() * 4);
+    put(h.o_src_j, h.src_j.data(), h.src_j.size() * 4);
+    put(h.o_src_i, h.src_i.data(), h.src_i.size() * 4);
+    put(h.o_src_val, h.src_val.data(), h.src_val.size() * 8);
+    put(h.o_src_off, h.src_off.data(), h.src_off.size() * 4);
+    put(h.o_src_len, h.src_len.data(), h.src_len.size() * 4);
+    put(h.o_ovr_begin, h.ovr_begin.data(), h.ovr_begin.size() * 4);
+    put(h.o_ovr_j, h.ovr_j.data(), h.ovr_j.size() * 4);
+    put(h.o_ovr_i, h.ovr_i.data(), h.ovr_i.size() * 4);
+    put(h.o_ovr_val, h.ovr_val.data(), h.ovr_val.size() * 8);
+    put(h.o_ovr_off, h.ovr_off.data(), h.ovr_off.size() * 4);
+    put(h.o_ovr_len, h.ovr_len.data(), h.ovr_len.size() * 4);
+    put(h.o_series_t, d->series_t, (size_t)h.n_series * 8);
+    put(h.o_series_v, d->series_v, (size_t)h.n_series * 8);
+    put(h.o_rain_t, d->rain_t, (size_t)h.n_rain * 8);
+    put(h.o_rain_r, d->rain_r, (size_t)h.n_rain * 8);
 
     Ctl c{};
     memset(&c, 0, sizeof(c));
@@ -598,21 +746,23 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     c.t[cur ^ 1] = 0.0;
     c.next_mass[cur ^ 1] = d->mass_int;
     c.par[0] = c.par[1] = parity;  // the initial state sits in buffer `parity`
-    if ((rc = up(h.o_ctl, &c, sizeof(c)))) return rc;
+    put(h.o_ctl, &c, sizeof(c));
+    AF_HIP(hipMemcpyAsync(A + aux_lo, aux.data(), aux.size(), hipMemcpyHostToDevice, st));
+    // the host sources (aux, the staged inputs) must outlive the copies
     AF_HIP(hipStreamSynchronize(st));
     return AF_OK;
 }
 
 template <typename T, int V, bool F64, int OCC, int PF>
-int launch_step_v(af_domain* D, int kmod6, hipStream_t st) {
+int launch_step_v(af_domain* D, const TileDesc* tiles, int ntiles, int kmod6, hipStream_t st) {
     const int wpb = D->wpb;
-    const unsigned blocks = (unsigned)((D->ntiles + wpb - 1) / wpb);
+    const unsigned blocks = (unsigned)((ntiles + wpb - 1) / wpb);
     if (wpb == 4)
         hipLaunchKernelGGL((k_step<T, V, F64, 4, OCC, PF>), dim3(blocks), dim3(256), 0, st,
-                           D->d_scns, D->d_tiles, D->ntiles, kmod6);
+                           D->d_scns, tiles, ntiles, kmod6);
     else
         hipLaunchKernelGGL((k_step<T, V, F64, 1, OCC, PF>), dim3(blocks), dim3(64), 0, st,
-                           D->d_scns, D->d_tiles, D->ntiles, kmod6);
+                           D->d_scns, tiles, ntiles, kmod6);
     AF_HIP(hipGetLastError());
     return AF_OK;
 }
@@ -620,24 +770,32 @@ int launch_step_v(af_domain* D, int kmod6, hipStream_t st) {
 // Kernel variants (af_tuning.variant - 1): rows per loop step (PF = 1: one
 // row, the next in flight; PF = 2: two-row load and store bursts) and the
 // occupancy (waves per SIMD) the register allocation targets.
-int launch_step(af_domain* D, int kmod6, hipStream_t st) {
-    if (D->ntiles == 0) return AF_OK;
+int launch_tiles(af_domain* D, const TileDesc* t, int n, int kmod6, hipStream_t st) {
+    if (n == 0) return AF_OK;
     if (D->precision == AF_FP64) {
         switch (D->variant) {
-            case 1: return launch_step_v<double, 2, true, 2, 2>(D, kmod6, st);
-            case 2: return launch_step_v<double, 2, true, 1, 2>(D, kmod6, st);
-            default: return launch_step_v<double, 2, true, 2, 1>(D, kmod6, st);
+            case 1: return launch_step_v<double, 2, true, 2, 2>(D, t, n, kmod6, st);
+            case 2: return launch_step_v<double, 2, true, 1, 2>(D, t, n, kmod6, st);
+            default: return launch_step_v<double, 2, true, 2, 1>(D, t, n, kmod6, st);
         }
     }
     switch (D->variant) {
-        case 1: return launch_step_v<float, 4, false, 3, 2>(D, kmod6, st);
-        case 2: return launch_step_v<float, 4, false, 2, 2>(D, kmod6, st);
-        default: return launch_step_v<float, 4, false, 4, 1>(D, kmod6, st);
+        case 1: return launch_step_v<float, 4, false, 3, 2>(D, t, n, kmod6, st);
+        case 2: return launch_step_v<float, 4, false, 2, 2>(D, t, n, kmod6, st);
+        default: return launch_step_v<float, 4, false, 4, 1>(D, t, n, kmod6, st);
     }
 }
 
-int build_graph(af_domain* D, hipStream_t st) {
-    if (D->gexec) return AF_OK;
+// One step of every active scenario.
+int launch_step(af_domain* D, int kmod6, hipStream_t st) {
+    return launch_tiles(D, D->d_tiles, D->ntiles, kmod6, st);
+}
+
+// Captures `steps` launches over the tile table (tiles, n), with step phases
+// 0, 1, 2, ... (mod 6); `steps` is a multiple of 6, so a replay leaves every
+// scenario's rotating slots where it found them.
+int capture_steps(af_domain* D, const TileDesc* tiles, int n, int steps, hipGraph_t* gout,
+                  hipGraphExec_t* eout) {
     hipStream_t cs;
     AF_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
@@ -647,7 +805,7 @@ int build_graph(af_domain* D, hipStream_t st) {
         return AF_E_HIP;
     }
     int rc = AF_OK;
-    for (int i = 0; i < D->graph_steps && rc == AF_OK; i++) rc = launch_step(D, i % 6, cs);
+    for (int i = 0; i < steps && rc == AF_OK; i++) rc = launch_tiles(D, tiles, n, i % 6, cs);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -664,10 +822,22 @@ int build_graph(af_domain* D, hipStream_t st) {
         set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
         return AF_E_HIP;
     }
-    D->graph = g;
-    D->gexec = ge;
-    (void)st;
+    *gout = g;
+    *eout = ge;
     return AF_OK;
+}
+
+void drop_graph(hipGraph_t& g, hipGraphExec_t& e) {
+    if (e) (void)hipGraphExecDestroy(e);
+    if (g) (void)hipGraphDestroy(g);
+    e = nullptr;
+    g = nullptr;
+}
+
+int build_graph(af_domain* D, hipStream_t st) {
+    (void)st;
+    if (D->gexec) return AF_OK;
+    return capture_steps(D, D->d_tiles, D->ntiles, D->graph_steps, &D->graph, &D->gexec);
 }
 
 // Reduction of sum(H*dA) and wet area for af_domain_volume.
@@ -735,7 +905,11 @@ __global__ void k_gather_ctl(Ctl* const* __restrict__ list, int n, Ctl* __restri
 
 // The control blocks of the active scenarios (slots in D->active order),
 // read with one gather launch, one copy and one synchronisation.
+int settle(af_domain* D, hipStream_t st);
+
 int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
+    int rc = settle(D, st);
+    if (rc) return rc;
     const int n = (int)D->active.size();
     out.resize(n);
     if (n == 0) return AF_OK;
@@ -754,10 +928,51 @@ bool slot_finished(const af_domain* D, int s, const Ctl& c, int par) {
     return !(c.t[par] < D->scn[s].dev.sim_time) || c.status[par] != 0;
 }
 
+// The step index whose phase (mod 6) slot s's next step takes: the pool's,
+// or 0 for a scenario in the lane, which entered the lane at a pool step
+// index that is a multiple of 6 and steps there in multiples of 6.
+long long scn_k(const af_domain* D, int s) { return D->scn[s].in_lane ? 0 : D->k; }
+
+// Makes `st` wait for the lane work enqueued so far: every call that reads
+// or changes scenario state or the tile tables does this first.
+hipStream_t joined(af_domain* D, af_stream_t stream) {
+    hipStream_t st = to_stream(stream);
+    if (D->lane_pending) {
+        if (hipStreamWaitEvent(st, D->ev_join, 0) != hipSuccess)
+            (void)hipStreamSynchronize(D->lane_st);
+        D->lane_pending = false;
+    }
+    return st;
+}
+
 const char* status_text(int status) {
     if (status & kStatusNonfinite) return "a step produced a NaN or infinite depth";
     if (status & kStatusStalled) return "the clock stalled (t + dt == t)";
     return "unknown status";
+}
+
+// The lane's tile table: every tile of the lane's active scenarios, listed as
+// the full table lists them.  Released slots leave the lane.  The lane's
+// captured graph holds the tile count, so it is dropped.
+int build_lane(af_domain* D, hipStream_t st) {
+    std::vector<int> keep;
+    for (int s : D->lane_slots)
+        if (s >= 0 && s < D->max_scn && D->scn[s].active && D->scn[s].in_lane) keep.push_back(s);
+    D->lane_slots.swap(keep);
+    std::vector<TileDesc> tiles;
+    for (int s : D->lane_slots) {
+        const HostScn& h = D->scn[s];
+        for (int seg = 0; seg < h.nsegs; seg++)
+            for (int strip = 0; strip < h.nstrips; strip++)
+                tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
+    }
+    if (!tiles.empty())
+        AF_HIP(hipMemcpyAsync(D->d_lane, tiles.data(), sizeof(TileDesc) * tiles.size(),
+                              hipMemcpyHostToDevice, st));
+    AF_HIP(hipStreamSynchronize(st));
+    D->nlane = (int)tiles.size();
+    drop_graph(D->lgraph, D->lgexec);
+    return AF_OK;
 }
 
 // Rebuilds and uploads the tile table from the active scenarios; a captured
@@ -771,6 +986,7 @@ int rebuild_tiles(af_domain* D, hipStream_t st) {
         if (!h.active) continue;
         act.push_back(s);
         ctls.push_back(h.dev.ctl);
+        if (h.in_lane) continue;    // stepped by the lane
         for (int seg = 0; seg < h.nsegs; seg++)
             for (int strip = 0; strip < h.nstrips; strip++)
                 tiles.push_back({s, strip, seg, (strip == 0 && seg == 0) ? 1 : 0});
@@ -802,15 +1018,22 @@ int rebuild_tiles(af_domain* D, hipStream_t st) {
     AF_HIP(hipStreamSynchronize(st));
     D->ntiles = (int)tiles.size();
     D->active = std::move(act);
-    if (D->gexec) {
-        (void)hipGraphExecDestroy(D->gexec);
-        D->gexec = nullptr;
+    D->dirty = false;
+    drop_graph(D->graph, D->gexec);
+    return build_lane(D, st);
+}
+
+// Admissions and releases mark the tile tables stale; the next call that
+// launches steps or gathers the control blocks rebuilds them once.
+// (Lane work still in flight reads the lane's table: the stream waits for it
+// first.)
+int settle(af_domain* D, hipStream_t st) {
+    if (!D->dirty) return AF_OK;
+    if (D->lane_pending) {
+        AF_HIP(hipStreamWaitEvent(st, D->ev_join, 0));
+        D->lane_pending = false;
     }
-    if (D->graph) {
-        (void)hipGraphDestroy(D->graph);
-        D->graph = nullptr;
-    }
-    return AF_OK;
+    return rebuild_tiles(D, st);
 }
 
 // Re-sorts one point list (sources or overrides) of scenario h for its
@@ -924,6 +1147,7 @@ int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn,
     if (arena_bytes > tables) D->free_list.push_back({tables, arena_bytes - tables});
     D->d_scns = reinterpret_cast<ScnDev*>(D->arena + D->o_scns);
     D->d_tiles = reinterpret_cast<TileDesc*>(D->arena + D->o_tiles);
+    D->d_lane = reinterpret_cast<TileDesc*>(D->arena + D->o_lane);
     D->d_volbuf = reinterpret_cast<double*>(D->arena + D->o_volbuf);
     D->d_ctlptr = reinterpret_cast<Ctl**>(D->arena + D->o_ctlptr);
     D->d_ctlbuf = reinterpret_cast<Ctl*>(D->arena + D->o_ctlbuf);
@@ -933,9 +1157,17 @@ int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn,
 
 // Places a scenario in a free slot and heap extent and uploads it; its state
 // enters at the pool's current step index k.
-int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool rebuild, int* slot) {
-    int rc = validate(d);
-    if (rc) return rc;
+// Admits scenario d into a free slot.  `pre` holds its inputs staged by
+// stage_inputs (which also validated d); without it they are staged here.
+int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool rebuild, int* slot,
+               const Staged* pre = nullptr) {
+    int rc = AF_OK;
+    Staged local;
+    if (!pre) {
+        rc = stage_inputs(d, D->precision, local);
+        if (rc) return rc;
+        pre = &local;
+    }
     int s = -1;
     for (int i = 0; i < D->max_scn; i++)
         if (!D->scn[i].active) { s = i; break; }
@@ -956,7 +1188,7 @@ int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool reb
     D->scn[s] = std::move(h);
     HostScn& hs = D->scn[s];
     const int parity = (int)(D->k & 1);
-    rc = upload_scenario(D, s, d, st, parity, D->k % 6);
+    rc = upload_scenario(D, s, d, st, parity, D->k % 6, *pre);
     if (rc) {
         heap_free(D, off, bytes);
         D->scn[s] = HostScn{};
@@ -966,10 +1198,7 @@ int pool_admit(af_domain* D, const af_scenario_desc* d, hipStream_t st, bool reb
     hs.active = true;
     AF_HIP(hipMemcpyAsync(D->d_scns + s, &hs.dev, sizeof(ScnDev), hipMemcpyHostToDevice, st));
     AF_HIP(hipStreamSynchronize(st));
-    if (rebuild) {
-        rc = rebuild_tiles(D, st);
-        if (rc) return rc;
-    }
+    if (rebuild) D->dirty = true;
     *slot = s;
     return AF_OK;
 }
@@ -1085,11 +1314,38 @@ int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, in
 int af_pool_admit(af_domain* D, const af_scenario_desc* desc, af_stream_t stream, int32_t* slot) {
     AF_REQUIRE(D && desc && slot, AF_E_INVALID, "admit: bad arguments");
     int s = -1;
-    int rc = pool_admit(D, desc, to_stream(stream), true, &s);
+    int rc = pool_admit(D, desc, joined(D, stream), true, &s);
     if (rc) return rc;
     *slot = s;
     return AF_OK;
 }
+
+int af_pool_stage(const af_scenario_desc* desc, int precision, af_staged** out) {
+    AF_REQUIRE(desc && out, AF_E_INVALID, "stage: bad arguments");
+    *out = nullptr;
+    af_staged* p = new (std::nothrow) af_staged();
+    AF_REQUIRE(p, AF_E_CAPACITY, "stage: out of host memory");
+    int rc = stage_inputs(desc, precision, p->s);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    p->desc = *desc;
+    *out = p;
+    return AF_OK;
+}
+
+int af_pool_admit_staged(af_domain* D, const af_staged* staged, af_stream_t stream,
+                         int32_t* slot) {
+    AF_REQUIRE(D && staged && slot, AF_E_INVALID, "admit_staged: bad arguments");
+    int s = -1;
+    int rc = pool_admit(D, &staged->desc, joined(D, stream), true, &s, &staged->s);
+    if (rc) return rc;
+    *slot = s;
+    return AF_OK;
+}
+
+void af_staged_free(af_staged* staged) { delete staged; }
 
 int af_pool_release(af_domain* D, int32_t slot, af_stream_t stream) {
     AF_REQUIRE(D && slot >= 0 && slot < D->max_scn && D->scn[slot].active, AF_E_INVALID,
@@ -1097,26 +1353,26 @@ int af_pool_release(af_domain* D, int32_t slot, af_stream_t stream) {
     HostScn& h = D->scn[slot];
     heap_free(D, h.base, h.bytes);
     h = HostScn{};
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     if (D->pool) {
         int rc = retile_on_drain(D, st);
         if (rc) return rc;
     }
-    return rebuild_tiles(D, st);
+    D->dirty = true;
+    return AF_OK;
 }
 
 int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finished,
                  int32_t* n_active, af_stream_t stream) {
     AF_REQUIRE(D && n_finished, AF_E_INVALID, "poll: bad arguments");
-    hipStream_t st = to_stream(stream);
-    const int par = (int)(D->k & 1);
+    hipStream_t st = joined(D, stream);
     std::vector<Ctl> c;
     int rc = gather_ctl(D, c, st);
     if (rc) return rc;
     int nf = 0;
     for (size_t a = 0; a < D->active.size(); a++) {
         const int s = D->active[a];
-        if (slot_finished(D, s, c[a], par)) {
+        if (slot_finished(D, s, c[a], (int)(scn_k(D, s) & 1))) {
             if (finished && nf < cap) finished[nf] = s;
             nf++;
         }
@@ -1129,14 +1385,14 @@ int af_pool_poll(af_domain* D, int32_t* finished, int32_t cap, int32_t* n_finish
 int af_pool_progress(af_domain* D, int32_t* slots, double* t, int64_t* steps, int32_t* status,
                      int32_t* finished, int32_t cap, int32_t* n, af_stream_t stream) {
     AF_REQUIRE(D && n && cap >= 0, AF_E_INVALID, "progress: bad arguments");
-    hipStream_t st = to_stream(stream);
-    const int par = (int)(D->k & 1);
+    hipStream_t st = joined(D, stream);
     std::vector<Ctl> c;
     int rc = gather_ctl(D, c, st);
     if (rc) return rc;
     const int na = (int)D->active.size();
     for (int a = 0; a < na && a < cap; a++) {
         const int s = D->active[a];
+        const int par = (int)(scn_k(D, s) & 1);
         if (slots) slots[a] = s;
         if (t) t[a] = c[a].t[par];
         if (steps) steps[a] = c[a].n_done;
@@ -1150,10 +1406,13 @@ int af_pool_progress(af_domain* D, int32_t* slots, double* t, int64_t* steps, in
 int af_pool_stats(af_domain* D, int32_t* active, int32_t* tiles, size_t* heap_free,
                   int64_t* step_index) {
     AF_REQUIRE(D, AF_E_INVALID, "stats: null pool");
-    int na = 0;
-    for (const HostScn& h : D->scn) na += h.active ? 1 : 0;
+    int na = 0, nt = 0;
+    for (const HostScn& h : D->scn) {
+        na += h.active ? 1 : 0;
+        nt += h.active ? h.ntiles : 0;
+    }
     if (active) *active = na;
-    if (tiles) *tiles = D->ntiles;
+    if (tiles) *tiles = nt;
     if (heap_free) *heap_free = heap_free_bytes(D);
     if (step_index) *step_index = D->k;
     return AF_OK;
@@ -1161,15 +1420,90 @@ int af_pool_stats(af_domain* D, int32_t* active, int32_t* tiles, size_t* heap_fr
 
 int af_domain_destroy(af_domain* D) {
     if (!D) return AF_OK;
-    if (D->gexec) (void)hipGraphExecDestroy(D->gexec);
-    if (D->graph) (void)hipGraphDestroy(D->graph);
+    if (D->lane_st) {
+        (void)hipStreamSynchronize(D->lane_st);
+        (void)hipEventDestroy(D->ev_fork);
+        (void)hipEventDestroy(D->ev_t0);
+        (void)hipEventDestroy(D->ev_join);
+        (void)hipStreamDestroy(D->lane_st);
+    }
+    drop_graph(D->graph, D->gexec);
+    drop_graph(D->lgraph, D->lgexec);
     delete D;
+    return AF_OK;
+}
+
+int af_pool_set_lane(af_domain* D, const int32_t* slots, int32_t n, af_stream_t stream) {
+    AF_REQUIRE(D && n >= 0 && (n == 0 || slots), AF_E_INVALID, "set_lane: bad arguments");
+    AF_REQUIRE(D->k % 6 == 0, AF_E_STATE,
+               "set_lane: the pool's step index %lld is not a multiple of 6", (long long)D->k);
+    std::vector<int> v;
+    for (int i = 0; i < n; i++) {
+        const int s = slots[i];
+        AF_REQUIRE(s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
+                   "set_lane: slot %d is not in use", s);
+        if (std::find(v.begin(), v.end(), s) == v.end()) v.push_back(s);
+    }
+    std::sort(v.begin(), v.end());
+    if (v == D->lane_slots) return AF_OK;
+    hipStream_t st = joined(D, stream);
+    // a scenario leaving or entering the lane stands at phase 0 on both sides
+    for (HostScn& h : D->scn) h.in_lane = false;
+    for (int s : v) D->scn[s].in_lane = true;
+    D->lane_slots.swap(v);
+    return rebuild_tiles(D, st);
+}
+
+int af_pool_advance_lane(af_domain* D, int64_t steps, int use_graph, af_stream_t stream) {
+    AF_REQUIRE(D && steps >= 0 && steps % 6 == 0, AF_E_INVALID,
+               "advance_lane: steps must be a non-negative multiple of 6");
+    hipStream_t st = to_stream(stream);
+    int src = settle(D, st);
+    if (src) return src;
+    if (D->nlane == 0 || steps == 0) return AF_OK;
+    if (!D->lane_st) {
+        AF_HIP(hipStreamCreateWithFlags(&D->lane_st, hipStreamNonBlocking));
+        AF_HIP(hipEventCreateWithFlags(&D->ev_fork, hipEventDisableTiming));
+        AF_HIP(hipEventCreate(&D->ev_t0));
+        AF_HIP(hipEventCreate(&D->ev_join));
+    }
+    if (use_graph && !D->lgexec) {
+        int rc = capture_steps(D, D->d_lane, D->nlane, D->graph_steps, &D->lgraph, &D->lgexec);
+        if (rc) return rc;
+    }
+    // the lane starts behind everything enqueued on `st` so far (table
+    // uploads, admissions) and runs beside what is enqueued there next
+    AF_HIP(hipEventRecord(D->ev_fork, st));
+    AF_HIP(hipStreamWaitEvent(D->lane_st, D->ev_fork, 0));
+    AF_HIP(hipEventRecord(D->ev_t0, D->lane_st));
+    int64_t left = steps;
+    while (use_graph && left >= D->graph_steps) {
+        AF_HIP(hipGraphLaunch(D->lgexec, D->lane_st));
+        left -= D->graph_steps;
+    }
+    for (int64_t i = 0; i < left; i++) {
+        int rc = launch_tiles(D, D->d_lane, D->nlane, (int)(i % 6), D->lane_st);
+        if (rc) return rc;
+    }
+    AF_HIP(hipEventRecord(D->ev_join, D->lane_st));
+    D->lane_pending = true;
+    D->lane_timed = true;
+    return AF_OK;
+}
+
+int af_pool_lane_ms(af_domain* D, float* ms) {
+    AF_REQUIRE(D && ms, AF_E_INVALID, "lane_ms: bad arguments");
+    AF_REQUIRE(D->lane_timed, AF_E_STATE, "lane_ms: no lane work has been enqueued");
+    AF_HIP(hipEventSynchronize(D->ev_join));
+    AF_HIP(hipEventElapsedTime(ms, D->ev_t0, D->ev_join));
     return AF_OK;
 }
 
 int af_domain_advance(af_domain* D, int64_t max_steps, int use_graph, af_stream_t stream) {
     AF_REQUIRE(D && max_steps >= 0, AF_E_INVALID, "advance: bad arguments");
     hipStream_t st = to_stream(stream);
+    int src = settle(D, st);
+    if (src) return src;
     if (use_graph) {
         int rc = build_graph(D, st);
         if (rc) return rc;
@@ -1192,31 +1526,30 @@ int af_domain_advance(af_domain* D, int64_t max_steps, int use_graph, af_stream_
 
 int af_batch_finished(af_domain* D, int* all_done, af_stream_t stream) {
     AF_REQUIRE(D && all_done, AF_E_INVALID, "finished: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     std::vector<Ctl> c;
     int rc = gather_ctl(D, c, st);
     if (rc) return rc;
-    const int par = (int)(D->k & 1);
     int done = 1;
     for (size_t a = 0; a < D->active.size(); a++)
-        if (!slot_finished(D, D->active[a], c[a], par)) done = 0;
+        if (!slot_finished(D, D->active[a], c[a], (int)(scn_k(D, D->active[a]) & 1))) done = 0;
     *all_done = done;
     return AF_OK;
 }
 
 int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t stream) {
     AF_REQUIRE(D && check_every > 0, AF_E_INVALID, "run: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     std::vector<Ctl> c;
     long long prev_done = -1;
     for (;;) {
         int rc = gather_ctl(D, c, st);
         if (rc) return rc;
-        const int par = (int)(D->k & 1);
         int done = 1;
         long long steps = 0;
         double tsum = 0.0;
         for (size_t a = 0; a < D->active.size(); a++) {
+            const int par = (int)(scn_k(D, D->active[a]) & 1);
             if (!slot_finished(D, D->active[a], c[a], par)) done = 0;
             steps += c[a].n_done;
             tsum += c[a].t[par];
@@ -1225,6 +1558,7 @@ int af_domain_run(af_domain* D, int64_t check_every, int use_graph, af_stream_t 
             // every scenario ran to its sim_time or stopped; name the first
             // that stopped
             for (size_t a = 0; a < D->active.size(); a++) {
+                const int par = (int)(scn_k(D, D->active[a]) & 1);
                 if (c[a].status[par] == 0) continue;
                 set_error("scenario %d stopped after step %lld at t = %.17g: %s", D->active[a],
                           (long long)c[a].n_done, c[a].t[par], status_text(c[a].status[par]));
@@ -1249,7 +1583,7 @@ int af_batch_set_sim_time(af_domain* D, int s, double sim_time, af_stream_t stre
     AF_REQUIRE(D && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "set_sim_time: bad arguments");
     AF_REQUIRE(sim_time == sim_time, AF_E_INVALID, "set_sim_time: NaN");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     D->scn[s].dev.sim_time = sim_time;
     // the kernel reads the table on every launch, graph replays included
     AF_HIP(hipMemcpyAsync(&D->d_scns[s].sim_time, &D->scn[s].dev.sim_time, sizeof(double),
@@ -1261,19 +1595,20 @@ int af_batch_set_sim_time(af_domain* D, int s, double sim_time, af_stream_t stre
 int af_batch_info(af_domain* D, int s, af_run_info* info, af_stream_t stream) {
     AF_REQUIRE(D && info && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "info: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     int rc = finalize_reports(D, s, st);
     if (rc) return rc;
     Ctl c;
     AF_HIP(hipMemcpyAsync(&c, D->scn[s].dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     AF_HIP(hipStreamSynchronize(st));
-    const int tpar = (int)(D->k & 1);  // every launched step writes t[(k+1) & 1]
+    const long long k = scn_k(D, s);
+    const int tpar = (int)(k & 1);  // every launched step writes t[(k+1) & 1]
     info->t = c.t[tpar];
     info->steps_done = c.n_done;
     info->last_dt = c.last_dt;
     info->n_reports = c.n_reports;
     info->finished = slot_finished(D, s, c, tpar) ? 1 : 0;
-    info->hmax = fkey_inv_d(c.hmax[(int)(D->k % 3)]);
+    info->hmax = fkey_inv_d(c.hmax[(int)(k % 3)]);
     info->status = c.status[tpar];
     info->reserved = 0;
     return AF_OK;
@@ -1287,7 +1622,7 @@ int af_batch_read(af_domain* D, int s, int which, double* out, af_stream_t strea
     AF_REQUIRE(D && out && s >= 0 && s < D->max_scn && D->scn[s].active && which >= 0 &&
                    which <= 4,
                AF_E_INVALID, "read: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     HostScn& h = D->scn[s];
     long long n_done = 0;
     const int par = current_parity(D, s, &n_done, st);
@@ -1337,7 +1672,7 @@ int af_batch_read_cells(af_domain* D, int s, const int64_t* idx, int n, double* 
     AF_REQUIRE(D && (n == 0 || (idx && out)) && n >= 0 && s >= 0 && s < D->max_scn &&
                    D->scn[s].active,
                AF_E_INVALID, "read_cells: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     HostScn& h = D->scn[s];
     const int par = current_parity(D, s, nullptr, st);
     if (par < 0) return par;
@@ -1421,11 +1756,11 @@ int af_batch_snapshot(af_domain* D, int s, double* H_out, double* Vx_out, double
     AF_REQUIRE(D && H_out && s >= 0 && s < D->max_scn && D->scn[s].active &&
                    ((Vx_out == nullptr) == (Vy_out == nullptr)),
                AF_E_INVALID, "snapshot: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     const HostScn& h = D->scn[s];
     const long long n = (long long)h.nx * h.ny;
     const int blocks = (int)std::min<long long>((n + 255) / 256, 16384);
-    const int kpar = (int)(D->k & 1);
+    const int kpar = (int)(scn_k(D, s) & 1);
     if (D->precision == AF_FP64)
         hipLaunchKernelGGL(k_snapshot<double>, dim3(blocks), dim3(256), 0, st, D->d_scns + s,
                            kpar, H_out, Vx_out, Vy_out);
@@ -1444,7 +1779,7 @@ int af_batch_read_dt_log(af_domain* D, int s, double* out, int64_t cap, int64_t*
                          af_stream_t stream) {
     AF_REQUIRE(D && n && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "dt_log: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     long long done = 0;
     if (current_parity(D, s, &done, st) < 0) return AF_E_HIP;
     long long avail = std::min<long long>(done, D->dt_log_cap);
@@ -1464,7 +1799,7 @@ int af_batch_read_reports(af_domain* D, int s, double* out, int32_t cap, int32_t
                           af_stream_t stream) {
     AF_REQUIRE(D && n && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "reports: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     int rc = finalize_reports(D, s, st);
     if (rc) return rc;
     Ctl c;
@@ -1486,7 +1821,7 @@ int af_domain_read_reports(af_domain* D, double* out, int32_t cap, int32_t* n,
 int af_batch_volume(af_domain* D, int s, double* vol, double* area, af_stream_t stream) {
     AF_REQUIRE(D && s >= 0 && s < D->max_scn && D->scn[s].active, AF_E_INVALID,
                "volume: bad arguments");
-    hipStream_t st = to_stream(stream);
+    hipStream_t st = joined(D, stream);
     HostScn& h = D->scn[s];
     const int par = current_parity(D, s, nullptr, st);
     if (par < 0) return par;
@@ -1539,7 +1874,9 @@ int af_batch_size(af_domain* D, int* n_scenarios, int* n_tiles) {
 
 int af_batch_launch_step(af_domain* D, af_stream_t stream) {
     AF_REQUIRE(D, AF_E_INVALID, "launch_step: null");
-    int rc = launch_step(D, (int)(D->k % 6), to_stream(stream));
+    int rc = settle(D, to_stream(stream));
+    if (rc) return rc;
+    rc = launch_step(D, (int)(D->k % 6), to_stream(stream));
     if (rc) return rc;
     D->k++;
     return AF_OK;

@@ -14,8 +14,9 @@ per step against sequential single-scenario runs), "served" (the served
 ensemble with its sequential baselines), "accuracy" (FP32/FP64 errors against
 FP64 references), "divergence" (lockstep drift at equal step counts), "shard"
 (one GPU's row slab), "driver" (the rebuilt reference program end to end),
-"lone" (step time of small and mid-size domains run alone) and "twin" (the
-FP64 path against itself with one input nudged by one ulp, beside FP32).
+"lone" (step time of small and mid-size domains run alone), "twin" (the
+FP64 path against itself with inputs nudged by one ulp, beside FP32) and
+"ceiling" (the bandwidth of the step's memory pattern without the step).
 
 Algorithmic bytes per cell-update: read z, H, Qx_old, Qy_old and write H, Qx,
 Qy = 7 values = 28 B at FP32 (56 B at FP64).
@@ -48,6 +49,27 @@ def _traffic_from_profile(precision):
     except OSError:
         pass
     return None
+
+
+def host_cpu():
+    """The host's CPU model and physical core count (lscpu), and nproc."""
+    import subprocess
+    model, cores, sockets = "unknown", None, None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k.strip() == "Model name":
+                model = v
+            elif k.strip() == "Core(s) per socket":
+                cores = int(v)
+            elif k.strip() == "Socket(s)":
+                sockets = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    phys = cores * sockets if cores and sockets else None
+    return {"model": model, "physical_cores": phys, "nproc": os.cpu_count()}
 
 
 def measure(scn, precision, steps, warmup, tile_rows, wpb, variant=0):
@@ -124,6 +146,32 @@ def sequential_fused(scns, precision, steps, warmup=3):
         cu += scn.cells * steps
         d.close()
     return cu / sec / 1e9, sec
+
+
+def sequential_fused_own(scns, own_steps, precision, steps, warmup=3):
+    """The same single-scenario timing as sequential_fused, weighted two ways:
+    uniform (every scenario `steps` steps, as sequential_fused) and by each
+    scenario's own step count (`own_steps`, e.g. what it took in a served
+    run), i.e. the time to run the scenarios to completion one after another
+    at their measured lone step times."""
+    import torch
+    from adaptflood_amd.domain import Domain, make_tuning
+    cu, sec, cu_own, sec_own = 0, 0.0, 0, 0.0
+    for scn, own in zip(scns, own_steps):
+        d = Domain(scn, precision=precision, tuning=make_tuning(graph_steps=36))
+        d.advance(warmup, graph=False)
+        d.advance(0, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.advance(steps, graph=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        d.close()
+        sec += dt
+        cu += scn.cells * steps
+        sec_own += dt / steps * own
+        cu_own += scn.cells * own
+    return cu / sec / 1e9, cu_own / sec_own / 1e9, sec_own
 
 
 def sequential_plain(scns, steps):
@@ -221,7 +269,7 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def run_served(scns, arrivals, prec, budget_gb, chunk):
+def run_served(scns, arrivals, prec, budget_gb, chunk, lane=False):
     """The continuous-batching loop over requests arriving at `arrivals` (s)."""
     import torch
     from adaptflood_amd import generators
@@ -235,7 +283,7 @@ def run_served(scns, arrivals, prec, budget_gb, chunk):
             for k in range(2)]
     serve(warm, pool, chunk_steps=36, clock="wall")
     reqs = [Request(k, s, float(a)) for k, (s, a) in enumerate(zip(scns, arrivals))]
-    done, summary = serve(reqs, pool, chunk_steps=chunk, clock="wall")
+    done, summary = serve(reqs, pool, chunk_steps=chunk, clock="wall", lane=lane)
     print(f"[bench] served {summary['requests']} in {summary['wall_s']:.1f}s: "
           f"{summary['gcups']:.1f} GCUPS, p50 {summary['p50_latency']:.2f}s, "
           f"p99 {summary['p99_latency']:.2f}s", file=sys.stderr, flush=True)
@@ -737,6 +785,108 @@ def bench_shard(args):
                     f"roofline case), {len(pts)} QFIX", "cells": cells}, **res}), flush=True)
 
 
+def bench_ceiling(args):
+    """The HBM bandwidth the step's memory pattern reaches with no stencil and
+    no arithmetic (af_hbm_probe, built from adaptflood_amd/csrc/hbm_probe.hip),
+    beside a read-only stream and a copy: the ceiling the roofline fraction of
+    the step is measured against."""
+    import subprocess
+    exe = os.path.join(ROOT, "adaptflood_amd", "_lib", "af_hbm_probe")
+    r = subprocess.run([exe, str(args.size), "64", "2"], capture_output=True, text=True,
+                       timeout=300)
+    sys.stderr.write(r.stdout)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        sys.exit(r.returncode)
+    best = {}
+    for line in r.stdout.splitlines():
+        f = line.split()
+        name, ms, gbs = f[0], float(f[1]), float(f[3])
+        if name not in best or gbs > best[name]["GB/s"]:
+            best[name] = {"ms": ms, "GB/s": gbs, "frac_of_8TB/s": round(gbs / HBM_PEAK_GBS, 4)}
+    print(json.dumps({"metric": "HBM bandwidth of the step's access pattern without the step",
+                      "grid": f"{args.size}x{args.size}, 256-column strips, 64-row tiles",
+                      "patterns": best}), flush=True)
+
+
+def measure_wet(scn, args):
+    """FP64 step time on the roofline workload once it has wetted: the domain
+    runs to --wet-time simulated seconds, then `steps` steps are timed (the
+    FP64 step evaluates the exact pow(h, 10/3) only in wavefronts with a wet
+    face, so it costs more as the flood spreads)."""
+    import torch
+    from adaptflood_amd.domain import Domain, make_tuning
+    d = Domain(scn, precision="fp64", tuning=make_tuning(graph_steps=216))
+    d.set_sim_time(args.wet_time)
+    d.run(check_every=2160, graph=True)
+    d.set_sim_time(scn.sim_time)
+    H = d.read("H")
+    wet = float(np.count_nonzero(H > 0.01)) / H.size
+    d.advance(0, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.advance(args.steps, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = _event_ms(torch, d.launch_step, min(args.steps, 50)) * 1e-3
+    info = d.info()
+    d.close()
+    cells = scn.cells
+    a64 = 56.0 * cells / kern / 1e9
+    print(f"[bench] fp64 wet (t={info['t']:.0f}s, {wet * 100:.1f}% of cells wet): "
+          f"{wall * 1e3 / args.steps:.3f} ms/step, {kern * 1e3:.3f} ms/launch, {a64:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    return {"value": round(cells * args.steps / wall / 1e9, 3), "unit": "GCUPS",
+            "ms_per_step": round(wall * 1e3 / args.steps, 4),
+            "state": f"after {args.wet_time:.0f} s of the run, {wet * 100:.1f}% of cells wet "
+                     f"(H > 0.01 m)",
+            "roofline": {"bound": "hbm", "achieved": round(a64, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(a64 / HBM_PEAK_GBS, 4),
+                         "kernel_ms": round(kern * 1e3, 4)}}
+
+
+def bench_batched_leg(scns, args):
+    """The 512-scenario ensemble in one launch per step (autotuned tiling)
+    against every 4th scenario run alone through the fused single-domain path."""
+    import torch
+    from adaptflood_amd.domain import Batch, make_tuning
+    from adaptflood_amd.tune import autotune
+    n, cells = len(scns), sum(s.cells for s in scns)
+    t = time.perf_counter()
+    best, _ = autotune(scns, "fp32", rows=(8, 16, 32, 64), launches=10)
+    tuning_s = time.perf_counter() - t
+    tun = make_tuning(tile_rows=best.tile_rows, waves_per_block=best.waves_per_block,
+                      variant=best.variant, graph_steps=36, tile_order=best.tile_order)
+    b = Batch(scns, precision="fp32", tuning=tun)
+    b.advance(args.warmup, graph=False)
+    b.advance(0, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(args.steps, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    gcups = cells * args.steps / wall / 1e9
+    sample = scns[::args.seq_stride]
+    g_seq, _ = sequential_fused(sample, "fp32", args.seq_steps)
+    print(f"[bench] batched ensemble: {gcups:.1f} GCUPS ({wall * 1e3 / args.steps:.3f} ms/step,"
+          f" rows {best.tile_rows} variant {best.variant}); sequential fused {g_seq:.2f} GCUPS",
+          file=sys.stderr, flush=True)
+    return {"value": round(gcups, 3), "unit": "GCUPS",
+            "ms_per_step": round(wall * 1e3 / args.steps, 4),
+            "workload": f"{n} ragged scenarios from the ensemble generator (seed 4000, sides "
+                        f"log-uniform 128..2048, aspect U(1,4), {cells} cells), FP32, one launch "
+                        f"per step, {args.steps} steps after {args.warmup}",
+            "tiling": {"tile_rows": best.tile_rows, "variant": best.variant,
+                       "tile_order": best.tile_order, "autotune_s": round(tuning_s, 1)},
+            "sequential_fused": {"value": round(g_seq, 3), "unit": "GCUPS",
+                                 "sample": f"every {args.seq_stride}th scenario ({len(sample)}), "
+                                           f"{args.seq_steps} steps each, alone"},
+            "speedup_vs_sequential_fused": round(gcups / g_seq, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -751,6 +901,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=80,
                     help="CPU oracle steps of the baseline sample (about 15 s on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="leave out the batched 512-scenario ensemble key")
+    ap.add_argument("--wet-time", type=float, default=1800.0,
+                    help="fp64_wet: simulated seconds run before timing FP64 (0: leave out)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the reference-call-sequence timing of the default workload")
     ap.add_argument("--no-served", action="store_true",
@@ -760,7 +914,7 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
                     choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard",
-                             "driver", "lone", "twin"),
+                             "driver", "lone", "twin", "ceiling"),
                     default="roofline")
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--twin-cases", default="roofline:2048:3600,valley:4096:7200,"
@@ -818,6 +972,14 @@ def main():
         return bench_lone(args)
     if args.workload == "twin":
         return bench_twin(args)
+    if args.workload == "ceiling":
+        return bench_ceiling(args)
+    batched_scns = None
+    if not args.no_batched:
+        t = time.perf_counter()
+        batched_scns = generate_members(list(range(args.members)), 4000, [3600.0] * args.members)
+        print(f"[bench] generated {len(batched_scns)} ensemble scenarios in "
+              f"{time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
     served_reqs = None
     if not args.no_served:
         # generated in worker processes before this process touches the GPU
@@ -895,6 +1057,10 @@ def main():
                                     "frac": round(a64 / HBM_PEAK_GBS, 4),
                                     "traffic": _traffic_from_profile("fp64"),
                                     "kernel_ms": round(kern64 * 1e3, 4)}}
+    if not args.no_fp64 and args.wet_time > 0:
+        out["fp64_wet"] = measure_wet(scn, args)
+    if batched_scns is not None:
+        out["batched"] = bench_batched_leg(batched_scns, args)
     if not args.no_plain:
         # the reference's own call sequence (per-function FP64 kernels, dt read
         # back every step) on the same grid, for scale
@@ -925,23 +1091,52 @@ def main():
         # draw leaves the per-request sim_times unchanged)
         from adaptflood_amd import generators as gen
         arr_sat, _ = gen.arrivals(len(reqs), rate=1000.0, seed=5)
-        done, summary, budget = run_served(reqs, arr_sat, "fp32", args.budget_gb, args.chunk)
+        done, summary, budget = run_served(reqs, arr_sat, "fp32", args.budget_gb, args.chunk,
+                                           lane=True)
+        # the same requests one after another through the single-scenario
+        # fused path: every seq_stride-th request, timed alone over
+        # seq_steps steps, aggregated uniformly and by its served step count
+        own = {r.rid: r.steps for r in done}
+        sample = [k for k in range(0, len(reqs), args.seq_stride) if k in own]
+        g_fused, g_own, s_own = sequential_fused_own(
+            [reqs[k] for k in sample], [own[k] for k in sample], "fp32", args.seq_steps)
+        print(f"[bench] sequential fused fp32 ({len(sample)} requests): {g_fused:.2f} GCUPS "
+              f"({args.seq_steps} steps each), {g_own:.2f} GCUPS (own step counts)",
+              file=sys.stderr, flush=True)
         out["served_saturated"] = {
-            "workload": f"the same {len(reqs)} requests, Poisson arrivals at 1000/s",
+            "workload": f"the same {len(reqs)} requests, Poisson arrivals at 1000/s, with the "
+                        f"pool's lane",
             "p50_latency_s": round(summary["p50_latency"], 3),
             "p99_latency_s": round(summary["p99_latency"], 3),
             "gcups": round(summary["gcups"], 3), "wall_s": round(summary["wall_s"], 3),
-            "arrival_span_s": round(float(arr_sat[-1]), 3)}
+            "arrival_span_s": round(float(arr_sat[-1]), 3),
+            "sequential_fused": {
+                "value": round(g_fused, 3), "unit": "GCUPS",
+                "sample": f"every {args.seq_stride}th request ({len(sample)}), alone, "
+                          f"{args.seq_steps} graph-replayed steps each"},
+            "sequential_fused_own_steps": {
+                "value": round(g_own, 3), "unit": "GCUPS",
+                "sample": "the same requests at their measured lone step times, each for the "
+                          "steps it took when served (run to completion one after another)",
+                "seconds": round(s_own, 3)},
+            "speedup_vs_sequential_fused": round(summary["gcups"] / g_fused, 2),
+            "speedup_vs_sequential_fused_own_steps": round(summary["gcups"] / g_own, 2)}
     if not args.no_cpu_baseline:
         from oracle import oracle
-        r = oracle.OracleRun(scn, dt_log_cap=args.cpu_steps + 1, mass_log_cap=4)
+        r = oracle.OracleRun(scn, dt_log_cap=args.cpu_steps + 11, mass_log_cap=4)
+        r.advance(10)                      # untimed warm-up steps
+        warm_s = r.step_seconds
         r.advance(args.cpu_steps)
-        cpu_gcups = cells * r.steps / r.step_seconds / 1e9
+        secs = r.step_seconds - warm_s
+        cpu_gcups = cells * args.cpu_steps / secs / 1e9
+        host = host_cpu()
         out["cpu_baseline"] = {
             "value": round(cpu_gcups, 4), "unit": "GCUPS", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": f"FP64 OpenMP oracle, first {r.steps} steps of the same "
-                      f"{scn.nx}x{scn.ny} workload ({r.step_seconds:.1f} s)"}
+            "sample": f"FP64 OpenMP oracle, steps 11-{10 + args.cpu_steps} of the same "
+                      f"{scn.nx}x{scn.ny} workload after 10 untimed steps ({secs:.1f} s)",
+            "host_cpu": host["model"], "host_physical_cores": host["physical_cores"],
+            "host_nproc": host["nproc"]}
         print(f"[bench] cpu: {cpu_gcups:.3f} GCUPS on {oracle.num_threads()} threads",
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)

@@ -327,6 +327,21 @@ int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, in
 /* AF_E_CAPACITY when no slot, tile-table room or heap extent is free. */
 int af_pool_admit(af_domain* pool, const af_scenario_desc* desc, af_stream_t stream,
                   int32_t* slot);
+/* Admission in two parts, for a server that admits while the pool steps:
+ * af_pool_stage does the host-only part (validation, and the terrain and
+ * initial depths converted to the device format of `precision`, in pinned
+ * host memory that is cached for reuse, never returned to the system while
+ * the process runs); it touches no pool and launches nothing, so it may run
+ * on any thread, while launches are in flight.  af_pool_admit_staged then does the rest, as af_pool_admit does
+ * (AF_E_CAPACITY when there is no room; the staged scenario may be admitted
+ * later, or into another pool of the same precision).  The arrays of `desc`
+ * must stay valid until the staged scenario is freed.  No counterpart in the
+ * reference, whose main.cpp:166-226 reads and uploads one scenario per run. */
+typedef struct af_staged af_staged; /* opaque, host side */
+int af_pool_stage(const af_scenario_desc* desc, int precision, af_staged** out);
+int af_pool_admit_staged(af_domain* pool, const af_staged* staged, af_stream_t stream,
+                         int32_t* slot);
+void af_staged_free(af_staged* staged);
 int af_pool_release(af_domain* pool, int32_t slot, af_stream_t stream);
 /* Slots whose clock reached their sim_time or that stopped with a non-zero
  * status (up to cap), and the active count. */
@@ -339,6 +354,24 @@ int af_pool_poll(af_domain* pool, int32_t* finished, int32_t cap, int32_t* n_fin
  * *n is the number of active slots. */
 int af_pool_progress(af_domain* pool, int32_t* slots, double* t, int64_t* steps, int32_t* status,
                      int32_t* finished, int32_t cap, int32_t* n, af_stream_t stream);
+/* The lane: the listed slots' tiles move from the pool's tile table to a
+ * second one (an empty list empties it; released slots leave it), at a pool
+ * step index that is a multiple of 6.  af_pool_advance_lane enqueues `steps`
+ * steps (a multiple of 6) of the lane's scenarios on a stream of the pool's
+ * own, from a captured graph of graph_steps launches when use_graph is set:
+ * they start behind the work enqueued on `stream` so far and run beside the
+ * af_domain_advance launches enqueued there next, which no longer step them.
+ * Every other call on the pool first makes `stream` wait for that lane work.
+ * A scenario that needs many more steps than the others (a small time step)
+ * so takes extra steps at the cost of its own small launches, overlapped with
+ * the pool's.  Each scenario keeps its own dt and clock, and its results are
+ * those it gives alone (the phase of its rotating slots is kept: it enters
+ * and leaves the lane at phase 0 and steps there in multiples of 6).
+ * af_pool_lane_ms: duration of the last lane work (waits for it to end).
+ * No counterpart in the reference, which runs one scenario per process. */
+int af_pool_set_lane(af_domain* pool, const int32_t* slots, int32_t n, af_stream_t stream);
+int af_pool_advance_lane(af_domain* pool, int64_t steps, int use_graph, af_stream_t stream);
+int af_pool_lane_ms(af_domain* pool, float* ms);
 int af_pool_stats(af_domain* pool, int32_t* active, int32_t* tiles, size_t* heap_free,
                   int64_t* step_index);
 

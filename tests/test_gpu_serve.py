@@ -244,3 +244,121 @@ def test_bad_scenarios_are_isolated_in_the_pool(gpu, prec):
         with pytest.raises(AFError, match="stalled"):
             d.run(check_every=6, graph=True)
         assert d.info()["status"] == AF_SCN_STALLED and d.info()["steps"] == 1
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_lane_steps_equal_alone(gpu, prec):
+    """The pool's lane (af_pool_set_lane / af_pool_advance_lane): the lane's
+    scenarios step on the lane's own stream, before, beside or after the
+    pool's launches, the lane's members change mid-run, and every scenario's
+    step count and depth grid still equal its lone run bitwise.  A lane burst
+    that is not a multiple of 6 steps, or a lane change where the pool's step
+    index is not a multiple of 6, is refused."""
+    from adaptflood_amd._lib import AFError
+    scns = [generators.small_case(96, 40, seed=160, dx=2.0, sim_time=60.0),
+            generators.small_case(64, 64, seed=161, sim_time=120.0),
+            generators.small_case(130, 33, seed=162, dx=3.0, sim_time=90.0),
+            generators.small_case(40, 90, seed=163, sim_time=150.0)]
+    tun = make_tuning(tile_rows=16)
+    pool = Pool(budget_bytes=8 << 20, precision=prec, max_scenarios=4, max_tiles=512, tuning=tun)
+    slots = [pool.admit(s) for s in scns]
+    assert None not in slots
+    with pytest.raises(AFError):
+        pool.advance_lane(8)
+    pool.advance(3, graph=False)
+    with pytest.raises(AFError):
+        pool.set_lane([slots[0]])              # the pool stands at step 3
+    pool.advance(3, graph=False)
+    pool.set_lane([slots[0]])
+    left = set(slots)
+    steps = {}
+    for it in range(20000):
+        if it == 40:
+            pool.set_lane([slots[0], slots[2]])
+        if it == 90:
+            pool.set_lane([slots[2]])
+        if it % 2:
+            pool.advance_lane(18, graph=(it % 3 != 0))     # beside the chunk
+            pool.advance(12, graph=(it % 4 == 1))
+        else:
+            pool.advance(12, graph=(it % 4 == 0))
+            pool.advance_lane(18, graph=(it % 3 != 0))     # behind it
+        fin, _, _ = pool.poll()
+        if it == 1:
+            assert pool.lane_ms() > 0.0
+        for s in fin:
+            if s in left:
+                steps[s] = pool.info(s)["steps"]
+                left.discard(s)
+        if not left:
+            break
+    assert not left
+    for s, scn in zip(slots, scns):
+        d = Domain(scn, precision=prec, tuning=tun)
+        d.run(check_every=24, graph=True)
+        print(f"{prec} {scn.name}: {steps[s]} steps", flush=True)
+        assert steps[s] == d.info()["steps"]
+        assert np.array_equal(pool.read(s, "H"), d.read("H")), scn.name
+        assert np.array_equal(pool.read(s, "Qx"), d.read("Qx")), scn.name
+        d.close()
+
+
+def test_served_with_lane_equals_alone(gpu):
+    """serve() puts a request whose time step is a fifth of the others' in the
+    lane; every result still equals its lone run bitwise."""
+    scns = [generators.small_case(64, 48, seed=170 + k, sim_time=120.0) for k in range(9)]
+    scns.append(generators.small_case(64, 48, seed=179, dx=2.0, sim_time=120.0))
+    tun = make_tuning(tile_rows=16)
+
+    class Watched(Pool):
+        lanes = []
+
+        def set_lane(self, slots):
+            self.lanes.append(tuple(slots))
+            super().set_lane(slots)
+
+    reqs = [Request(k, s, arrival=0) for k, s in enumerate(scns)]
+    pool = Watched(budget_bytes=16 << 20, precision="fp32", max_scenarios=10, max_tiles=1024,
+                   tuning=tun)
+    done, summary = serve(reqs, pool, chunk_steps=12, clock="steps", keep_results=True,
+                          lane=True)
+    assert summary["requests"] == len(scns) and summary["failed"] == 0
+    assert any(len(l) for l in Watched.lanes)
+    for r in done:
+        d = Domain(r.scenario, precision="fp32", tuning=tun)
+        d.run(check_every=24, graph=True)
+        assert r.steps == d.info()["steps"], r.rid
+        assert np.array_equal(r.result, d.read("H")), r.rid
+        d.close()
+
+
+def test_staged_admission_equals_alone(gpu):
+    """Scenarios staged on worker threads (af_pool_stage) and admitted with
+    af_pool_admit_staged step exactly as when admitted directly; a scenario
+    staged for the other precision is refused."""
+    import concurrent.futures
+    from adaptflood_amd.serve import RequestError, Staged
+    from adaptflood_amd._lib import AF_FP64
+    scns = [generators.small_case(90, 40 + 9 * k, seed=190 + k, sim_time=60.0) for k in range(4)]
+    tun = make_tuning(tile_rows=16)
+    pool = Pool(budget_bytes=8 << 20, precision="fp32", max_scenarios=4, max_tiles=512, tuning=tun)
+    with concurrent.futures.ThreadPoolExecutor(4) as ex:
+        staged = list(ex.map(pool.stage, scns))
+    with pytest.raises(RequestError, match="does not match"):
+        pool.admit(Staged(scns[0], AF_FP64))
+    slots = [pool.admit(st) for st in staged]
+    assert None not in slots and all(st.handle is None for st in staged)
+    left = set(slots)
+    for _ in range(5000):
+        pool.advance(12, graph=True)
+        fin, _, _ = pool.poll()
+        left -= set(fin)
+        if not left:
+            break
+    assert not left
+    for s, scn in zip(slots, scns):
+        d = Domain(scn, precision="fp32", tuning=tun)
+        d.run(check_every=24, graph=True)
+        assert pool.info(s)["steps"] == d.info()["steps"]
+        assert np.array_equal(pool.read(s, "H"), d.read("H")), scn.name
+        d.close()

@@ -2,7 +2,9 @@
 stand-in pool that simulates admission capacity and step counts on the CPU:
 admission order, waiting while full, release and re-admission, completion
 times and the latency summary, and the isolation of bad requests (refused at
-admission, stopped on the device, or with a clock that stalls)."""
+admission, stopped on the device, or with a clock that stalls), and the lane
+that lets a request needing many more steps than the others take extra steps
+between chunks."""
 from adaptflood_amd._lib import AF_SCN_NONFINITE
 from adaptflood_amd.serve import Request, RequestError, serve
 
@@ -53,7 +55,7 @@ class FakePool:
     def poll(self):
         done = [s for s, (scn, k) in sorted(self.active.items())
                 if k >= scn.steps or self._status(scn, k)]
-        clocks = {s: self._clock(scn, k) for s, (scn, k) in self.active.items()}
+        clocks = {s: (self._clock(scn, k), k) for s, (scn, k) in self.active.items()}
         return done, len(self.active), clocks
 
     def info(self, slot):
@@ -151,3 +153,89 @@ def test_bad_requests_are_isolated_and_the_rest_complete():
     # every admitted request was released, so the pool is empty again
     assert not pool.active
     assert sum(1 for e in pool.log if e[0] == "release") == 4
+
+
+class LaneScn:
+    def __init__(self, cells, dt, sim_time):
+        self.cells, self.dt, self.sim_time = cells, dt, sim_time
+        self.steps = int(round(sim_time / dt))
+        self.stall_at = self.fail_at = None
+        self.bad = False
+
+
+class LanePool(FakePool):
+    """FakePool with a lane: set_lane(slots) moves slots out of the pool's
+    steps (at a pool step that is a multiple of 6) and advance_lane(steps)
+    steps only them, in multiples of 6."""
+
+    def __init__(self, slots):
+        super().__init__(slots)
+        self.lane, self.lane_steps = [], 0
+
+    def _clock(self, scn, k):
+        return k * scn.dt if hasattr(scn, "dt") else super()._clock(scn, k)
+
+    def set_lane(self, slots):
+        assert all(s in self.active for s in slots) and self.step % 6 == 0
+        self.lane = list(slots)
+        self.log.append(("lane", self.step, tuple(slots)))
+
+    def advance(self, steps, graph=True):
+        self.step += steps
+        for s, rec in self.active.items():
+            if s not in self.lane:
+                rec[1] = min(rec[0].steps, rec[1] + steps)
+
+    def advance_lane(self, steps, graph=True):
+        assert steps > 0 and steps % 6 == 0
+        for s in self.lane:
+            rec = self.active[s]
+            rec[1] = min(rec[0].steps, rec[1] + steps)
+        self.lane_steps += steps
+
+    def lane_ms(self):
+        return 0.0
+
+    def release(self, slot):
+        super().release(slot)
+        self.lane = [s for s in self.lane if s != slot]
+
+
+def test_lane_takes_the_slow_request_ahead_of_the_pool():
+    """Nine requests of 600 steps and one of 6000 (a tenth of the time step):
+    with the lane, the slow one takes several chunks of steps per chunk while
+    the others run, so it ends after far fewer pool launches than its own
+    6000 steps; every request still runs exactly its own number of steps, the
+    fast ones are never in the lane and end when they did without it."""
+    def run(lane):
+        scns = [LaneScn(100, 1.0, 600.0) for _ in range(9)] + [LaneScn(100, 0.1, 600.0)]
+        reqs = [Request(k, s, arrival=0) for k, s in enumerate(scns)]
+        pool = LanePool(slots=10)
+        done, summary = serve(reqs, pool, chunk_steps=36, clock="steps", lane=lane)
+        assert summary["requests"] == 10 and summary["failed"] == 0
+        assert all(r.steps == scns[r.rid].steps for r in done)
+        return reqs, pool
+
+    plain, pool0 = run(False)
+    laned, pool1 = run(True)
+    assert pool0.lane_steps == 0 and not any(e[0] == "lane" for e in pool0.log)
+    assert pool1.lane_steps > 0
+    assert all(e[2] in ((), (9,)) for e in pool1.log if e[0] == "lane")
+    assert plain[9].done >= 6000
+    assert pool1.lane_steps >= 4 * 36 * (600 // 36 - 2)   # from the second poll on
+    assert laned[9].done <= 0.6 * plain[9].done
+    # the fast requests end when they did without the lane
+    assert [r.done for r in laned[:9]] == [r.done for r in plain[:9]]
+
+
+def test_lane_holds_no_more_than_its_share_of_cells():
+    """Two slow requests, each a fifth of the active cells: with the lane
+    capped at a tenth of the cells neither joins; at a half both do."""
+    scns = [LaneScn(100, 1.0, 600.0) for _ in range(3)] + [LaneScn(100, 0.1, 600.0)] * 2
+    for share, joined in ((0.1, set()), (0.5, {3, 4})):
+        reqs = [Request(k, s, arrival=0) for k, s in enumerate(scns)]
+        pool = LanePool(slots=5)
+        done, _ = serve(reqs, pool, chunk_steps=36, clock="steps", lane=True, lane_cells=share)
+        got = set(s for e in pool.log if e[0] == "lane" for s in e[2])
+        assert got == joined
+        assert all(r.steps == scns[r.rid].steps for r in done)
