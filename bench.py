@@ -309,10 +309,14 @@ def bench_served(args):
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
     prec = args.precision
-    done, summary, budget = run_served(scns, arr, prec, args.budget_gb, args.chunk)
-    sample = scns[::args.seq_stride]
-    g_fused, s_fused = sequential_fused(sample, prec, args.seq_steps)
-    print(f"[bench] sequential fused {prec} ({len(sample)} scenarios): {g_fused:.2f} GCUPS",
+    done, summary, budget = run_served(scns, arr, prec, args.budget_gb, args.chunk,
+                                       lane=args.lane)
+    own = {r.rid: r.steps for r in done}
+    pick = [k for k in range(0, n, args.seq_stride) if k in own]
+    g_fused, g_own, s_own = sequential_fused_own([scns[k] for k in pick], [own[k] for k in pick],
+                                                 prec, args.seq_steps)
+    print(f"[bench] sequential fused {prec} ({len(pick)} scenarios): {g_fused:.2f} GCUPS "
+          f"({args.seq_steps} steps each), {g_own:.2f} GCUPS (own step counts)",
           file=sys.stderr, flush=True)
     plain_sample = scns[::args.plain_stride]
     g_plain, s_plain = sequential_plain(plain_sample, args.plain_steps)
@@ -327,7 +331,8 @@ def bench_served(args):
         "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": f"{n} requests, Poisson arrivals at {args.rate}/s, ensemble "
                                f"generator (seed 5), sim_time U(600,3600) s, HBM budget "
-                               f"{budget / (1 << 30):.0f} GiB, chunks of {args.chunk} steps",
+                               f"{budget / (1 << 30):.0f} GiB, chunks of {args.chunk} steps"
+                               f"{', with the lane' if args.lane else ''}",
                    "requests": n, "cells": cells, "precision": prec},
         "p50_latency_s": round(summary["p50_latency"], 3),
         "p99_latency_s": round(summary["p99_latency"], 3),
@@ -335,13 +340,18 @@ def bench_served(args):
         "arrival_span_s": round(float(arr[-1]), 3),
         "cell_updates": int(summary["cell_updates"]),
         "sequential_fused": {"value": round(g_fused, 3), "unit": "GCUPS",
-                             "sample": f"every {args.seq_stride}th request ({len(sample)}), "
+                             "sample": f"every {args.seq_stride}th request ({len(pick)}), "
                                        f"{args.seq_steps} steps each"},
+        "sequential_fused_own_steps": {
+            "value": round(g_own, 3), "unit": "GCUPS",
+            "sample": "the same requests at their measured lone step times, each for the steps "
+                      "it took when served", "seconds": round(s_own, 3)},
         "sequential_plain": {"value": round(g_plain, 3), "unit": "GCUPS",
                              "sample": f"every {args.plain_stride}th request "
                                        f"({len(plain_sample)}), {args.plain_steps} steps each, "
                                        f"fp64 reference call sequence"},
         "speedup_vs_sequential_fused": round(summary["gcups"] / g_fused, 2),
+        "speedup_vs_sequential_fused_own_steps": round(summary["gcups"] / g_own, 2),
         "speedup_vs_sequential_plain": round(summary["gcups"] / g_plain, 2),
     }
     print(json.dumps(out), flush=True)
@@ -948,6 +958,9 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="served arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="served HBM budget (GiB)")
     ap.add_argument("--chunk", type=int, default=36, help="served steps per pool advance")
+    ap.add_argument("--lane", action="store_true",
+                    help="served: move requests that need many more steps than the rest to the "
+                         "pool's lane (the default bench's saturated leg always does)")
     ap.add_argument("--autotune", action="store_true",
                     help="ensemble: pick tile rows and kernel variant by timing candidates")
     args = ap.parse_args()

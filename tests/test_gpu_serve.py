@@ -251,16 +251,20 @@ def test_lane_steps_equal_alone(gpu, prec):
     """The pool's lane (af_pool_set_lane / af_pool_advance_lane): the lane's
     scenarios step on the lane's own stream, before, beside or after the
     pool's launches, the lane's members change mid-run, and every scenario's
-    step count and depth grid still equal its lone run bitwise.  A lane burst
-    that is not a multiple of 6 steps, or a lane change where the pool's step
-    index is not a multiple of 6, is refused."""
+    step count, depth grid, discharges and mass reports still equal its lone
+    run bitwise.  A lane burst that is not a multiple of 6 steps, or a lane
+    change where the pool's step index is not a multiple of 6, is refused."""
+    import dataclasses
     from adaptflood_amd._lib import AFError
+    from adaptflood_amd.domain import Batch
     scns = [generators.small_case(96, 40, seed=160, dx=2.0, sim_time=60.0),
             generators.small_case(64, 64, seed=161, sim_time=120.0),
-            generators.small_case(130, 33, seed=162, dx=3.0, sim_time=90.0),
+            generators.small_case(700, 200, seed=162, dx=3.0, sim_time=90.0),
             generators.small_case(40, 90, seed=163, sim_time=150.0)]
+    scns = [dataclasses.replace(s, mass_int=5.0) for s in scns]
     tun = make_tuning(tile_rows=16)
-    pool = Pool(budget_bytes=8 << 20, precision=prec, max_scenarios=4, max_tiles=512, tuning=tun)
+    pool = Pool(budget_bytes=64 << 20, precision=prec, max_scenarios=4, max_tiles=512,
+                tuning=tun)
     slots = [pool.admit(s) for s in scns]
     assert None not in slots
     with pytest.raises(AFError):
@@ -274,9 +278,11 @@ def test_lane_steps_equal_alone(gpu, prec):
     steps = {}
     for it in range(20000):
         if it == 40:
-            pool.set_lane([slots[0], slots[2]])
+            pool.set_lane([slots[0], slots[2], slots[3]])
         if it == 90:
             pool.set_lane([slots[2]])
+        if it == 150:
+            pool.set_lane([slots[1], slots[3]])
         if it % 2:
             pool.advance_lane(18, graph=(it % 3 != 0))     # beside the chunk
             pool.advance(12, graph=(it % 4 == 1))
@@ -300,6 +306,9 @@ def test_lane_steps_equal_alone(gpu, prec):
         assert steps[s] == d.info()["steps"]
         assert np.array_equal(pool.read(s, "H"), d.read("H")), scn.name
         assert np.array_equal(pool.read(s, "Qx"), d.read("Qx")), scn.name
+        assert np.array_equal(pool.read(s, "Qy"), d.read("Qy")), scn.name
+        rp, rd = Batch.reports(pool, s), d.reports()
+        assert len(rd) >= 10 and np.array_equal(rp, rd), scn.name
         d.close()
 
 
