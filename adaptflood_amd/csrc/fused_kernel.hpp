@@ -890,6 +890,12 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
     return x * q + min(x, r) + k;
 }
 
+// Static LDS of a k_step workgroup: each wave's batched point entries.
+template <typename T, int WPB>
+constexpr size_t step_static_lds() {
+    return (size_t)WPB * 2 * kPointBatch * (sizeof(int) + sizeof(T));
+}
+
 template <typename T, int V, bool F64, int WPB, int OCC, int PF>
 __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_step(const ScnDev* __restrict__ scns, const TileDesc* __restrict__ tiles, int ntiles, int kmod6) {
@@ -904,6 +910,7 @@ k_step(const ScnDev* __restrict__ scns, const TileDesc* __restrict__ tiles, int 
     // per wave: the batched source and override entries of the tile
     __shared__ int l_rc[WPB][2 * kPointBatch];
     __shared__ T l_val[WPB][2 * kPointBatch];
+    static_assert(sizeof(l_rc) + sizeof(l_val) == step_static_lds<T, WPB>(), "static LDS");
     const lptr<int> lrc = (lptr<int>)&l_rc[wv][0];
     const lptr<T> lval = (lptr<T>)&l_val[wv][0];
     if (td.strip > 0 && (td.strip + 1) * W < S.nx)
