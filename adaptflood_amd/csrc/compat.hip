@@ -6,6 +6,7 @@
 // af_device.hpp.
 #include <float.h>
 
+#include <algorithm>
 #include <atomic>
 
 #include "af_device.hpp"
@@ -155,30 +156,13 @@ k_bcs(double* __restrict__ Qx, double* __restrict__ Qy, const double* __restrict
     }
 }
 
-// ApplyPointBCs (kernels.cpp:261-274).  A point writes only when no later
-// point targets the same cell, so duplicates resolve to the last in order.
-__global__ void __launch_bounds__(kThreads)
-k_apply_point_bcs(double* __restrict__ H, const int32_t* __restrict__ idx,
-                  const int32_t* __restrict__ type, const double* __restrict__ value, int n) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
-        int c = idx[b];
-        int ty = type[b];
-        if (c < 0 || (ty != 1 && ty != 2 && ty != 3)) continue;
-        bool last = true;
-        for (int o = b + 1; o < n; o++) {
-            int to = type[o];
-            if (idx[o] == c && (to == 1 || to == 2 || to == 3)) { last = false; break; }
-        }
-        if (!last) continue;
-        H[c] = (ty == 1) ? 0.0 : value[b];
-    }
-}
-
-// Point lists of up to kSortCap entries are grouped by cell with one sort in
-// LDS: key = cell << 32 | array position, so a stable order within a cell is
-// the array order.  The first entry of each run of equal cells then applies
-// the whole run, in array order.  O(n log^2 n) work in one workgroup, against
-// the n^2 compares of the scan kernels above, which remain for longer lists.
+// Point lists are grouped by cell with one sort in LDS per chunk of up to
+// kSortCap entries: key = cell << 32 | position in the chunk, so a stable
+// order within a cell is the array order.  The first entry of each run of
+// equal cells then applies the whole run, in array order.  Longer lists are
+// taken chunk after chunk in array order on one stream, so every cell still
+// sees its entries in array order: O(n log^2 kSortCap) work in all, where a
+// scan for each entry's duplicates would be O(n^2).
 constexpr int kSortCap = 8192;       // 64 KiB of keys in LDS
 constexpr int kSortThreads = 1024;
 constexpr unsigned long long kNoKey = ~0ull;
@@ -210,7 +194,7 @@ __device__ __forceinline__ int pow2_at_least(int n) {
     return m;
 }
 
-// AddFlowBCs (main.cpp:256-260) for n <= kSortCap sources.
+// AddFlowBCs (main.cpp:256-260) for one chunk of n <= kSortCap sources.
 __global__ void __launch_bounds__(kSortThreads)
 k_add_flow_sorted(double* __restrict__ H, const int32_t* __restrict__ idx,
                   const double* __restrict__ ps, int n) {
@@ -236,8 +220,8 @@ k_add_flow_sorted(double* __restrict__ H, const int32_t* __restrict__ idx,
     }
 }
 
-// ApplyPointBCs (kernels.cpp:261-274) for n <= kSortCap points: the last
-// FREE/HFIX/HVAR point of each cell in array order wins.
+// ApplyPointBCs (kernels.cpp:261-274) for one chunk of n <= kSortCap points:
+// the last FREE/HFIX/HVAR point of each cell in array order wins.
 __global__ void __launch_bounds__(kSortThreads)
 k_apply_point_bcs_sorted(double* __restrict__ H, const int32_t* __restrict__ idx,
                          const int32_t* __restrict__ type, const double* __restrict__ value,
@@ -293,29 +277,6 @@ k_rainfall(double* __restrict__ H, const double* __restrict__ DEM, long long n, 
             if (h < 0.0) h = 0.0;
             H[t] = h;
         }
-    }
-}
-
-// AddFlowBCs (main.cpp:256-260).  The first source of each cell applies all
-// of that cell's sources in array order; later duplicates do nothing.
-__global__ void __launch_bounds__(kThreads)
-k_add_flow(double* __restrict__ H, const int32_t* __restrict__ idx,
-           const double* __restrict__ ps, int n) {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        int c = idx[s];
-        if (c < 0) continue;
-        bool first = true;
-        for (int o = 0; o < s; o++) {
-            if (idx[o] == c) { first = false; break; }
-        }
-        if (!first) continue;
-        double h = H[c];
-        for (int o = s; o < n; o++) {
-            if (idx[o] != c) continue;
-            h += ps[o];
-            if (h < 0.0) h = 0.0;
-        }
-        H[c] = h;
     }
 }
 
@@ -501,13 +462,12 @@ int af_ApplyPointBCs_GPU(double* H, const int32_t* bc_idx, const int32_t* bc_typ
     if (num_bcs == 0) return AF_OK;
     AF_REQUIRE(H && bc_idx && bc_type && bc_value, AF_E_INVALID, "ApplyPointBCs: null pointer");
     hipStream_t s = to_stream(stream);
-    if (num_bcs <= kSortCap)
-        hipLaunchKernelGGL(k_apply_point_bcs_sorted, dim3(1), dim3(kSortThreads), 0, s, H, bc_idx,
-                           bc_type, bc_value, num_bcs);
-    else
-        hipLaunchKernelGGL(k_apply_point_bcs, dim3(blocks_for(num_bcs, kThreads)), dim3(kThreads),
-                           0, s, H, bc_idx, bc_type, bc_value, num_bcs);
-    AF_HIP(hipGetLastError());
+    for (int off = 0; off < num_bcs; off += kSortCap) {
+        const int m = std::min(kSortCap, num_bcs - off);
+        hipLaunchKernelGGL(k_apply_point_bcs_sorted, dim3(1), dim3(kSortThreads), 0, s, H,
+                           bc_idx + off, bc_type + off, bc_value + off, m);
+        AF_HIP(hipGetLastError());
+    }
     AF_HIP(hipStreamSynchronize(s));
     return AF_OK;
 }
@@ -548,13 +508,12 @@ int af_AddFlowBCs_GPU(double* H, const int32_t* ps_idx, const double* ps_flow, i
     if (num_flow_bcs == 0) return AF_OK;
     AF_REQUIRE(H && ps_idx && ps_flow, AF_E_INVALID, "AddFlowBCs: null pointer");
     hipStream_t s = to_stream(stream);
-    if (num_flow_bcs <= kSortCap)
-        hipLaunchKernelGGL(k_add_flow_sorted, dim3(1), dim3(kSortThreads), 0, s, H, ps_idx,
-                           ps_flow, num_flow_bcs);
-    else
-        hipLaunchKernelGGL(k_add_flow, dim3(blocks_for(num_flow_bcs, kThreads)), dim3(kThreads), 0,
-                           s, H, ps_idx, ps_flow, num_flow_bcs);
-    AF_HIP(hipGetLastError());
+    for (int off = 0; off < num_flow_bcs; off += kSortCap) {
+        const int m = std::min(kSortCap, num_flow_bcs - off);
+        hipLaunchKernelGGL(k_add_flow_sorted, dim3(1), dim3(kSortThreads), 0, s, H, ps_idx + off,
+                           ps_flow + off, m);
+        AF_HIP(hipGetLastError());
+    }
     AF_HIP(hipStreamSynchronize(s));
     return AF_OK;
 }

@@ -125,11 +125,12 @@ def test_sequence_plane_case(gpu):
     assert err <= 1e-9
 
 
-@pytest.mark.parametrize("n", [1, 2, 5000, 8192, 9000])
+@pytest.mark.parametrize("n", [1, 2, 5000, 8192, 9000, 20000])
 def test_point_lists_with_duplicates(gpu, n):
-    """AddFlowBCs_GPU and ApplyPointBCs_GPU on long lists (up to 8192 entries
-    grouped by one sort in LDS, longer ones by the scan kernels), with cells
-    repeated up to five times and negative depths to clamp: bitwise the
+    """AddFlowBCs_GPU and ApplyPointBCs_GPU on long lists (chunks of up to
+    8192 entries, each grouped by one sort in LDS, taken in array order; a
+    cell's entries fall in several chunks when the list is longer), with
+    cells repeated several times and negative depths to clamp: bitwise the
     oracle's sequential file-order loops (main.cpp:256-260, kernels.cpp:261-274)."""
     nx, ny = 300, 211
     rng = np.random.default_rng(1000 + n)
