@@ -789,14 +789,17 @@ int launch_step_v(af_domain* D, const TileDesc* tiles, int ntiles, int kmod6, hi
 
 // Kernel variants (af_tuning.variant - 1): rows per loop step (PF = 1: one
 // row, the next in flight; PF = 2: two-row load and store bursts) and the
-// occupancy (waves per SIMD) the register allocation targets.
+// occupancy (waves per SIMD) the register allocation targets and the launch
+// holds (occupancy_lds).  The one-row FP64 variant runs at three waves per
+// SIMD, which serves many short tiles (the ensembles); the two-row FP64
+// default at two, which serves the large domains.
 int launch_tiles(af_domain* D, const TileDesc* t, int n, int kmod6, hipStream_t st) {
     if (n == 0) return AF_OK;
     if (D->precision == AF_FP64) {
         switch (D->variant) {
             case 1: return launch_step_v<double, 2, true, 2, 2>(D, t, n, kmod6, st);
             case 2: return launch_step_v<double, 2, true, 1, 2>(D, t, n, kmod6, st);
-            default: return launch_step_v<double, 2, true, 2, 1>(D, t, n, kmod6, st);
+            default: return launch_step_v<double, 2, true, 3, 1>(D, t, n, kmod6, st);
         }
     }
     switch (D->variant) {

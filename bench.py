@@ -15,8 +15,11 @@ ensemble with its sequential baselines), "accuracy" (FP32/FP64 errors against
 FP64 references), "divergence" (lockstep drift at equal step counts), "shard"
 (one GPU's row slab), "driver" (the rebuilt reference program end to end),
 "lone" (step time of small and mid-size domains run alone), "twin" (the
-FP64 path against itself with inputs nudged by one ulp, beside FP32) and
-"ceiling" (the bandwidth of the step's memory pattern without the step).
+FP64 path against itself with inputs nudged by one ulp, beside FP32),
+"ceiling" (the bandwidth of the step's memory pattern without the step),
+"configs" (the plane 256^2 and valley 4096^2 workloads at FP32 and FP64
+beside the CPU oracle) and "budget" (on the CPU: the volume budget of the
+FP64 twin pair, sources, edge flow and depth-clamp gains, with the oracle).
 
 Algorithmic bytes per cell-update: read z, H, Qx_old, Qy_old and write H, Qx,
 Qy = 7 values = 28 B at FP32 (56 B at FP64).
@@ -630,6 +633,76 @@ def bench_twin(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_budget(args):
+    """Volume budget of the FP64 twin pair, on the CPU: the FP64 oracle runs
+    the roofline generator at --budget-size and its twin (every valid terrain
+    value + 1 ulp) for --budget-steps steps each, with the build that counts
+    the depth its clamps at zero add.  Each step's volume change splits into
+    the QFIX inflow (v*dx*dt per source, main.cpp:246-253), the net inflow
+    through the edge faces (their Q after BCs, times dt) and the clamp gains
+    (kernels.cpp:287); the difference between the two runs' volumes then
+    splits into the differences of those parts."""
+    from adaptflood_amd import generators
+    from oracle import oracle
+    base = generators.roofline_case(n=args.budget_size)
+    twin, how = _nudged(base, "dem")
+    nx, ny, dA = base.nx, base.ny, base.dx * base.dx
+    qfix = sum(p.value for p in base.points)
+    L = oracle.lib(clamp=True)
+    marks = []
+    m = 25
+    while m < args.budget_steps:
+        marks.append(m)
+        m *= 2
+    marks.append(args.budget_steps)
+
+    def edge_inflow(r):
+        qx = r.Qx.reshape(-1, nx + 1)
+        qy = r.Qy.reshape(-1, nx + 1)
+        return (float(qx[:ny, 0].sum()) - float(qx[:ny, nx].sum()) + float(qy[0, :nx].sum())
+                - float(qy[ny, :nx].sum()))
+
+    runs = {}
+    for name, scn in (("base", base), ("twin", twin)):
+        r = oracle.OracleRun(scn, dt_log_cap=1, mass_log_cap=1, clamp=True)
+        L.orc_clamp_gain(1)
+        v0 = float(r.H.sum()) * dA
+        acc = {"src": 0.0, "edge": 0.0, "clamp": 0.0}
+        rows = []
+        t0 = time.perf_counter()
+        while r.steps < args.budget_steps and r.t < scn.sim_time:
+            t_prev = r.t
+            r.advance(1)
+            dt = r.t - t_prev
+            acc["src"] += qfix * base.dx * dt
+            acc["edge"] += edge_inflow(r) * dt
+            acc["clamp"] += L.orc_clamp_gain(1) * dA
+            if r.steps in marks:
+                v = float(r.H.sum()) * dA
+                rows.append({"steps": r.steps, "t": r.t, "V": v, **acc,
+                             "residual": (v - v0) - acc["src"] - acc["edge"] - acc["clamp"]})
+                print(f"[budget] {name} step {r.steps} t={r.t:.1f}: V={v:.6e} src={acc['src']:.6e} "
+                      f"edge={acc['edge']:.6e} clamp={acc['clamp']:.6e} residual "
+                      f"{rows[-1]['residual']:.2e} ({time.perf_counter() - t0:.0f}s)",
+                      file=sys.stderr, flush=True)
+        runs[name] = rows
+    diff = []
+    for a, b in zip(runs["base"], runs["twin"]):
+        d = {"steps": a["steps"], "rel_volume_diff": (b["V"] - a["V"]) / a["V"]}
+        for k in ("src", "edge", "clamp"):
+            d["d_" + k] = b[k] - a[k]
+        diff.append(d)
+        print(f"[budget] step {a['steps']}: twin - base: V {b['V'] - a['V']:.4e} "
+              f"({d['rel_volume_diff']:.2e}) = src {d['d_src']:.4e} + edge {d['d_edge']:.4e} "
+              f"+ clamp {d['d_clamp']:.4e}", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "volume budget of the FP64 twin pair (CPU oracle)",
+                      "config": {"workload": f"roofline generator {nx}x{ny}, {len(base.points)} "
+                                             f"QFIX, free edges; twin: {how}",
+                                 "steps": args.budget_steps},
+                      "base": runs["base"], "twin": runs["twin"], "difference": diff}),
+          flush=True)
+
+
 def _asc_body(path):
     """The grid lines of an ESRI-ASCII file (the six header lines dropped)."""
     with open(path, "rb") as f:
@@ -759,6 +832,74 @@ def bench_lone(args):
                       "config": {"workload": "wet flat grids and served-ensemble members, "
                                              "each alone, graph-replayed steps"},
                       "domains": rows}), flush=True)
+
+
+def bench_configs(args):
+    """The two smaller single-domain workloads, each at FP32 and FP64 beside
+    the FP64 CPU oracle on the host's cores:
+      plane 256^2 at 10 m (one QFIX, FREE east edge), the full 3600 s run on
+        the GPU (graph-replayed) and on the CPU;
+      valley 4096^2 at 5 m (2000 QVAR, rain, HFIX west edge), `--steps`
+        graph-replayed steps after `--warmup` on the GPU (from the dry
+        start), and on the CPU steps 11-40 after 10 untimed steps.
+    GCUPS count every cell of the grid; the roofline fraction uses the
+    step's algorithmic bytes (28 B FP32, 56 B FP64 per cell) over the
+    event-timed launch."""
+    import torch
+    from adaptflood_amd import generators
+    from adaptflood_amd.domain import Domain, make_tuning
+    from oracle import oracle
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    host = host_cpu()
+    out = {"metric": "GCUPS and HBM roofline fraction, single-domain workloads",
+           "cpu": {"threads": oracle.num_threads(), "host_cpu": host["model"],
+                   "host_physical_cores": host["physical_cores"]}, "rows": []}
+    plane = generators.plane_case()
+    for prec in ("fp32", "fp64"):
+        d = Domain(plane, precision=prec, tuning=make_tuning(graph_steps=216))
+        d.advance(0, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.run(check_every=2160, graph=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        steps = d.info()["steps"]
+        d.close()
+        row = {"workload": "plane 256x256, full 3600 s", "precision": prec, "steps": steps,
+               "seconds": round(wall, 4), "gcups": round(plane.cells * steps / wall / 1e9, 3)}
+        out["rows"].append(row)
+        print(f"[bench] configs {row}", file=sys.stderr, flush=True)
+    r = oracle.OracleRun(plane, dt_log_cap=1 << 12, mass_log_cap=64)
+    r.advance()
+    row = {"workload": "plane 256x256, full 3600 s", "precision": "fp64 CPU oracle",
+           "steps": r.steps, "seconds": round(r.step_seconds, 4),
+           "gcups": round(plane.cells * r.steps / r.step_seconds / 1e9, 4)}
+    out["rows"].append(row)
+    print(f"[bench] configs {row}", file=sys.stderr, flush=True)
+    del r
+    valley = generators.valley_case(n=4096)
+    for prec, B in (("fp32", 28.0), ("fp64", 56.0)):
+        wall, kern, _ = measure(valley, prec, args.steps, args.warmup, args.tile_rows, args.wpb)
+        row = {"workload": f"valley 4096x4096, steps {args.warmup + 1}-{args.warmup + args.steps}",
+               "precision": prec, "steps": args.steps, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+               "gcups": round(valley.cells * args.steps / wall / 1e9, 3),
+               "kernel_ms": round(kern * 1e3, 4),
+               "roofline_frac": round(B * valley.cells / kern / 1e9 / HBM_PEAK_GBS, 4)}
+        out["rows"].append(row)
+        print(f"[bench] configs {row}", file=sys.stderr, flush=True)
+    r = oracle.OracleRun(valley, dt_log_cap=64, mass_log_cap=4)
+    r.advance(10)
+    warm = r.step_seconds
+    r.advance(30)
+    secs = r.step_seconds - warm
+    row = {"workload": "valley 4096x4096, steps 11-40", "precision": "fp64 CPU oracle",
+           "steps": 30, "seconds": round(secs, 3),
+           "gcups": round(valley.cells * 30 / secs / 1e9, 4)}
+    out["rows"].append(row)
+    print(f"[bench] configs {row}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
 
 
 def bench_shard(args):
@@ -924,13 +1065,17 @@ def main():
                          "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--workload",
                     choices=("roofline", "ensemble", "served", "accuracy", "divergence", "shard",
-                             "driver", "lone", "twin", "ceiling"),
+                             "driver", "lone", "twin", "ceiling", "configs", "budget"),
                     default="roofline")
     ap.add_argument("--parts", type=int, default=8, help="shard: row slabs of the roofline case")
     ap.add_argument("--twin-cases", default="roofline:2048:3600,valley:4096:7200,"
                     "roofline:16384:3600", help="twin: case:size:sim_time list")
     ap.add_argument("--twin-nudge", choices=("source", "dem"), default="dem",
                     help="twin: one source value +1 ulp, or every DEM value +1 ulp")
+    ap.add_argument("--budget-size", type=int, default=2048,
+                    help="budget: side of the roofline-generator grid (CPU oracle)")
+    ap.add_argument("--budget-steps", type=int, default=1600,
+                    help="budget: steps of each oracle run")
     ap.add_argument("--twin-linear", type=int, default=3200,
                     help="twin: checkpoint spacing once the doubling schedule reaches it")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
@@ -985,6 +1130,10 @@ def main():
         return bench_lone(args)
     if args.workload == "twin":
         return bench_twin(args)
+    if args.workload == "configs":
+        return bench_configs(args)
+    if args.workload == "budget":
+        return bench_budget(args)
     if args.workload == "ceiling":
         return bench_ceiling(args)
     batched_scns = None

@@ -57,6 +57,45 @@
 #include <omp.h>
 #endif
 
+/* Clamp accounting.  Every clamp of a depth at zero (kernels.cpp:287,
+ * kernels.cpp:363, main.cpp:259) adds the depth it removes to the water.
+ * Built with -DORC_COUNT_CLAMP (the third library of oracle/Makefile), the
+ * oracle sums those depths, so that a run's volume budget closes:
+ *   V(t) - V(0) = sources + rain + edge inflow + clamp gains + overrides.
+ * The default build clamps the same way and counts nothing. */
+#ifdef ORC_COUNT_CLAMP
+static double orc_clamp_total = 0.0;
+#define ORC_CLAMP_AT_ZERO(h, acc)  \
+    do {                           \
+        if ((h) < 0.0) {           \
+            (acc) -= (h);          \
+            (h) = 0.0;             \
+        }                          \
+    } while (0)
+#define ORC_GAIN_REDUCTION reduction(+ : gain)
+#define ORC_GAIN_ADD(acc) (orc_clamp_total += (acc))
+#else
+#define ORC_CLAMP_AT_ZERO(h, acc)  \
+    do {                           \
+        if ((h) < 0.0) (h) = 0.0;  \
+    } while (0)
+#define ORC_GAIN_REDUCTION
+#define ORC_GAIN_ADD(acc) ((void)(acc))
+#endif
+
+/* The depth the clamps have added since the last reset (a sum of depths;
+ * times dA for a volume), or -1 in a build that does not count. */
+double orc_clamp_gain(int reset) {
+#ifdef ORC_COUNT_CLAMP
+    double g = orc_clamp_total;
+    if (reset) orc_clamp_total = 0.0;
+    return g;
+#else
+    (void)reset;
+    return -1.0;
+#endif
+}
+
 static inline double getmax(double a, double b) { return (a > b) ? a : b; } /* types.hpp:14 */
 static inline double getmin(double a, double b) { return (a < b) ? a : b; } /* types.hpp:15 */
 
@@ -289,16 +328,18 @@ void orc_apply_point_bcs(double* H, const int* bc_idx, const int* bc_type,
 void orc_update_h(double* H, const double* Qx, const double* Qy, int xsz, int ysz, double dA,
                   double Tstep) {
     const long P = xsz + 1;
-#pragma omp parallel for schedule(static)
+    double gain = 0.0;
+#pragma omp parallel for schedule(static) ORC_GAIN_REDUCTION
     for (int j = 0; j < ysz; j++) {
         for (int i = 0; i < xsz; i++) {
             long idx = i + (long)j * xsz;
             double Qx_in = Qx[i + (long)j * P], Qx_out = Qx[(i + 1) + (long)j * P];
             double Qy_in = Qy[i + (long)j * P], Qy_out = Qy[i + (long)(j + 1) * P];
             H[idx] += Tstep * (Qx_in - Qx_out + Qy_in - Qy_out) / dA;
-            if (H[idx] < 0.0) H[idx] = 0.0;
+            ORC_CLAMP_AT_ZERO(H[idx], gain);
         }
     }
+    ORC_GAIN_ADD(gain);
 }
 
 /* kernels.cpp:354-367 */
@@ -307,22 +348,26 @@ void orc_rainfall(double* H, const double* DEM, int xsz, int ysz, double rain_ra
     if (rain_rate <= 0.0) return;
     double rain_depth = rain_rate * Tstep;
     long n = (long)xsz * ysz;
-#pragma omp parallel for schedule(static)
+    double gain = 0.0;
+#pragma omp parallel for schedule(static) ORC_GAIN_REDUCTION
     for (long k = 0; k < n; k++) {
         if (DEM[k] != nodata) {
             H[k] += rain_depth;
-            if (H[k] < 0.0) H[k] = 0.0;
+            ORC_CLAMP_AT_ZERO(H[k], gain);
         }
     }
+    ORC_GAIN_ADD(gain);
 }
 
 /* main.cpp:256-260, applied in file order. */
 void orc_add_flow_bcs(double* H, const int* ps_idx, const double* ps_flow, int n) {
+    double gain = 0.0;
     for (int s = 0; s < n; s++) {
         int idx = ps_idx[s];
         H[idx] += ps_flow[s];
-        if (H[idx] < 0.0) H[idx] = 0.0;
+        ORC_CLAMP_AT_ZERO(H[idx], gain);
     }
+    ORC_GAIN_ADD(gain);
 }
 
 /* io.cpp:299-311 (interpolate_bc_value) and io.cpp:88-100 (interpolate_rain):

@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 # the same oracle with the C library's pow(hflow, 10.0/3.0) in the flux
 LIB_LIBM = os.path.join(HERE, "_build", "liboracle_libm.so")
+# the same oracle summing the depth its clamps at zero add (orc_clamp_gain)
+LIB_CLAMP = os.path.join(HERE, "_build", "liboracle_clamp.so")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -50,18 +52,20 @@ _libs = {}
 
 
 def build(force=False):
-    if force or not (os.path.exists(LIB) and os.path.exists(LIB_LIBM)):
+    if force or not all(os.path.exists(p) for p in (LIB, LIB_LIBM, LIB_CLAMP)):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-f",
                         os.path.join(HERE, "Makefile")], check=True)
     return LIB
 
 
-def lib(libm=False):
-    """The pinned oracle, or with libm=True the build whose flux takes the C
-    library's pow (for measuring what that rounding does to a run)."""
-    if libm not in _libs:
+def lib(libm=False, clamp=False):
+    """The pinned oracle; with libm=True the build whose flux takes the C
+    library's pow (for measuring what that rounding does to a run); with
+    clamp=True the build that sums the depth its clamps add (same results)."""
+    key = "clamp" if clamp else ("libm" if libm else "pinned")
+    if key not in _libs:
         build()
-        L = C.CDLL(LIB_LIBM if libm else LIB)
+        L = C.CDLL({"clamp": LIB_CLAMP, "libm": LIB_LIBM, "pinned": LIB}[key])
         V = C.c_void_p
         D = C.c_double
         I = C.c_int
@@ -88,8 +92,10 @@ def lib(libm=False):
         L.orc_run.restype = I
         L.orc_num_threads.restype = I
         L.orc_set_num_threads.argtypes = [I]
-        _libs[libm] = L
-    return _libs[libm]
+        L.orc_clamp_gain.argtypes = [I]
+        L.orc_clamp_gain.restype = D
+        _libs[key] = L
+    return _libs[key]
 
 
 def _p(a):
@@ -166,9 +172,10 @@ class _Packed:
 class OracleRun:
     """State of one oracle run; step it with advance()."""
 
-    def __init__(self, s, dt_log_cap=1 << 20, mass_log_cap=4096, libm=False):
+    def __init__(self, s, dt_log_cap=1 << 20, mass_log_cap=4096, libm=False, clamp=False):
         self.s = s
         self.libm = libm
+        self.clamp = clamp
         self.pk = _Packed(s)
         ny, nx = s.dem.shape
         self.H = np.zeros(nx * ny)
@@ -193,9 +200,10 @@ class OracleRun:
         ml = np.zeros(4 * self._mass_cap)
         info.dt_log, info.dt_log_cap = dtl.ctypes.data_as(_dp), self._dt_cap
         info.mass_log, info.mass_log_cap = ml.ctypes.data_as(_dp), self._mass_cap
-        rc = lib(self.libm).orc_run(C.byref(self.pk.case), _p(self.H), _p(self.pk.dem), _p(self.Qx),
-                           _p(self.Qy), _p(self.Qxold), _p(self.Qyold), self.t, self.next_mass,
-                           C.byref(info))
+        L = lib(self.libm, self.clamp)
+        rc = L.orc_run(C.byref(self.pk.case), _p(self.H), _p(self.pk.dem), _p(self.Qx),
+                       _p(self.Qy), _p(self.Qxold), _p(self.Qyold), self.t, self.next_mass,
+                       C.byref(info))
         if rc != 0:
             raise RuntimeError("oracle run failed")
         n = int(info.steps_done)

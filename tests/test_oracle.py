@@ -295,6 +295,79 @@ def test_closed_domain_conserves_mass():
     assert abs(v1 - v0) / v0 < 1e-13
 
 
+def _edge_inflow(Qx, Qy, nx, ny):
+    """Net volume rate into the grid through its edge faces (m3/s): the west
+    and north faces feed their cells, the east and south faces drain them
+    (UpdateH, kernels.cpp:281-288)."""
+    P = nx + 1
+    qx = Qx.reshape(-1, P)
+    qy = Qy.reshape(-1, P)
+    return (float(np.sum(qx[:ny, 0])) - float(np.sum(qx[:ny, nx])) + float(np.sum(qy[0, :nx]))
+            - float(np.sum(qy[ny, :nx])))
+
+
+@pytest.mark.parametrize("init_tstep,clamps", [(1.0, True), (10.0, False)])
+def test_clamp_volume_closes_the_source_budget(init_tstep, clamps):
+    """The setup of the reference's test_point_source.cpp: a 50 x 50 bowl
+    z = 0.001 r at 10 m, closed edges, one 0.1 m2/s QFIX source at the centre,
+    60 s.  With InitTstep = 1 (the test's value) the first steps take a large
+    dt in shallow water and over-drain the source cell; each clamp at zero
+    adds the depth it removes.  With InitTstep = 10 no clamp fires.  The build
+    that counts those depths (ORC_COUNT_CLAMP) gives the same states bitwise,
+    and its count closes the budget: V(t_end) = injected + clamp gains."""
+    n, dx, q = 50, 10.0, 0.1
+    yy, xx = np.mgrid[0:n, 0:n]
+    dem = 0.001 * np.hypot(xx - n // 2, yy - n // 2) * dx
+    s = S(dem, dx, sim_time=60.0, init_tstep=init_tstep, points=[P(n // 2, n // 2, 4, q)])
+    base = oracle.run(s)
+    r = oracle.OracleRun(s, clamp=True)
+    L = oracle.lib(clamp=True)
+    L.orc_clamp_gain(1)
+    r.advance()
+    gain = L.orc_clamp_gain(1) * dx * dx
+    assert np.array_equal(r.H.reshape(n, n), base["H"]) and r.steps == base["steps"]
+    injected = q * dx * r.t          # QFIX adds v*dx*dt per step (main.cpp:246-253)
+    vol = float(np.sum(r.H)) * dx * dx
+    assert (gain > 0.0) == clamps
+    assert abs(vol - (injected + gain)) <= 1e-12 * vol
+    print(f"InitTstep {init_tstep}: t_end {r.t:.3f} s, injected {injected:.4f} m3, clamps added "
+          f"{gain:.4f} m3 ({gain / injected:.1%}), volume {vol:.4f} m3")
+    # the default build counts nothing
+    assert oracle.lib().orc_clamp_gain(0) == -1.0
+
+
+def test_volume_budget_with_edges_and_rain():
+    """Free edges, rain and a draining source on rough terrain, stepped one
+    step at a time: every step's volume change equals its sources and rain,
+    the net inflow through the edge faces times dt, and the clamp gains, to
+    rounding.  A dropped term (the rain, an edge) or a wrong sign breaks it."""
+    rng = np.random.default_rng(11)
+    ny, nx, dx = 30, 40, 5.0
+    dem = np.cumsum(rng.uniform(-0.2, 0.25, (ny, nx)), axis=1) + rng.uniform(0, 0.3, (ny, nx))
+    rain_t, rain_r = np.array([0.0, 200.0]), np.array([2e-5, 2e-5])
+    pts = [P(20, 15, 4, 0.5), P(5, 25, 4, -0.3), P(33, 4, 4, 0.2)]
+    s = S(dem, dx, sim_time=400.0, init_tstep=2.0, w=True, e=True, n=True, s=True, points=pts,
+          rain_times=rain_t, rain_rates=rain_r, h0=np.full((ny, nx), 0.05))
+    r = oracle.OracleRun(s, clamp=True)
+    L = oracle.lib(clamp=True)
+    L.orc_clamp_gain(1)
+    dA = dx * dx
+    gains = 0.0
+    while r.t < s.sim_time:
+        v0, t0 = float(np.sum(r.H)) * dA, r.t
+        r.advance(1)
+        dt = r.t - t0
+        rain = np.interp(t0, rain_t, rain_r) * dt * dA * nx * ny
+        src = sum(p.value for p in pts if p.value > 0) * dx * dt
+        sink = sum(p.value for p in pts if p.value < 0) * dx * dt
+        edge = _edge_inflow(r.Qx, r.Qy, nx, ny) * dt
+        gain = L.orc_clamp_gain(1) * dA
+        gains += gain
+        dv = float(np.sum(r.H)) * dA - v0
+        assert abs(dv - (src + sink + rain + edge + gain)) <= 1e-9 * (abs(v0) + 1.0), r.steps
+    assert gains > 0.0 and r.steps > 20
+
+
 def test_transpose_symmetry():
     """Swapping the x and y axes of the inputs swaps Qx and Qy of the result
     (E and S edges share one formula; W differs from N in the reference)."""
