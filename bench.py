@@ -644,7 +644,8 @@ def bench_budget(args):
     splits into the differences of those parts."""
     from adaptflood_amd import generators
     from oracle import oracle
-    base = generators.roofline_case(n=args.budget_size)
+    kw = {"n_qfix": args.budget_qfix} if args.budget_qfix > 0 else {}
+    base = generators.roofline_case(n=args.budget_size, **kw)
     twin, how = _nudged(base, "dem")
     nx, ny, dA = base.nx, base.ny, base.dx * base.dx
     qfix = sum(p.value for p in base.points)
@@ -1076,6 +1077,9 @@ def main():
                     help="budget: side of the roofline-generator grid (CPU oracle)")
     ap.add_argument("--budget-steps", type=int, default=1600,
                     help="budget: steps of each oracle run")
+    ap.add_argument("--budget-qfix", type=int, default=0,
+                    help="budget: QFIX sources (0: the generator's 5000; 78 keeps the 16384^2 "
+                         "case's sources per cell at 2048^2)")
     ap.add_argument("--twin-linear", type=int, default=3200,
                     help="twin: checkpoint spacing once the doubling schedule reaches it")
     ap.add_argument("--valley-size", type=int, default=4096, help="accuracy: valley case side")
