@@ -272,15 +272,17 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def run_served(scns, arrivals, prec, budget_gb, chunk, lane=False):
+def run_served(scns, arrivals, prec, budget_gb, chunk, lane=False, variant=0):
     """The continuous-batching loop over requests arriving at `arrivals` (s)."""
     import torch
     from adaptflood_amd import generators
+    from adaptflood_amd.domain import make_tuning
     from adaptflood_amd.serve import Pool, Request, serve
     n = len(scns)
     free, _ = torch.cuda.mem_get_info()
     budget = min(int(budget_gb * (1 << 30)), int(free * 0.8))
-    pool = Pool(budget_bytes=budget, precision=prec, max_scenarios=n, max_tiles=1 << 20)
+    pool = Pool(budget_bytes=budget, precision=prec, max_scenarios=n, max_tiles=1 << 20,
+                tuning=make_tuning(variant=variant))
     # a few requests through a warm pool first, so kernel loading is not timed
     warm = [Request(-1 - k, generators.ensemble_member(k, seed_base=5000, sim_time=400.0), 0.0)
             for k in range(2)]
@@ -313,7 +315,7 @@ def bench_served(args):
         sys.exit(1)
     prec = args.precision
     done, summary, budget = run_served(scns, arr, prec, args.budget_gb, args.chunk,
-                                       lane=args.lane)
+                                       lane=args.lane, variant=args.variant)
     own = {r.rid: r.steps for r in done}
     pick = [k for k in range(0, n, args.seq_stride) if k in own]
     g_fused, g_own, s_own = sequential_fused_own([scns[k] for k in pick], [own[k] for k in pick],
@@ -1047,6 +1049,8 @@ def main():
     ap.add_argument("--size", type=int, default=16384)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--wpb", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="served: the pool's kernel variant (0: the library's default)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-fp64", action="store_true",
                     help="measure only the FP64 step (for profiling runs)")
