@@ -1054,8 +1054,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--only-fp64", action="store_true",
                     help="measure only the FP64 step (for profiling runs)")
-    ap.add_argument("--cpu-steps", type=int, default=80,
-                    help="CPU oracle steps of the baseline sample (about 15 s on 16 threads)")
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="CPU oracle steps of the baseline sample, after 10 untimed ones "
+                         "(about 18 s on 16 threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="leave out the batched 512-scenario ensemble key")
