@@ -151,7 +151,6 @@ struct af_staged {
 struct af_domain {
     int precision = AF_FP32;
     int V = 4, W = 256, R = 128, wpb = 4, graph_steps = 36, variant = 0;
-    bool variant_fixed = false;  // af_tuning.variant named one
     int R_fixed = 0;             // tile rows asked for (0: chosen per scenario)
     int tile_order = 0;          // af_tuning.tile_order
     int min_rows = 8;            // lowest default tile height (see kMinRowsAlone)
@@ -432,10 +431,16 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
     D->precision = precision;
     D->V = precision == AF_FP64 ? 2 : 4;
     D->W = 64 * D->V;
-    // Defaults measured on the 16384 x 16384 step: two-row bursts, 64-row
-    // tiles at FP32 and 32-row tiles at FP64.
+    // Defaults measured on the 16384 x 16384 roofline case: 64-row tiles with
+    // two-row bursts at FP32; 32-row tiles with the one-row loop at three
+    // waves per SIMD at FP64.  The FP64 two-row variant at two waves is
+    // faster while the flow is dry (3.10 against 3.30 ms per step over the
+    // first 110 steps) and slower once it wets (the exact pow(h, 10/3) runs
+    // in every wavefront with a wet face): over the full 3600 s run 3.55
+    // against 3.20 ms per step.  The one-row FP64 variant is also the faster
+    // on the 4096^2 valley, the ragged ensemble and the served pool.
     D->R = precision == AF_FP64 ? 32 : 64;
-    D->variant = 1;
+    D->variant = precision == AF_FP64 ? 0 : 1;
     D->wpb = 4;
     D->graph_steps = 36;
     D->dt_log_cap = 0;
@@ -448,10 +453,7 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
         if (tun->report_cap > 0) D->report_cap = tun->report_cap;
         AF_REQUIRE(tun->variant >= 0 && tun->variant <= 3, AF_E_INVALID,
                    "variant %d (0 = default, 1..3)", tun->variant);
-        if (tun->variant > 0) {
-            D->variant = tun->variant - 1;
-            D->variant_fixed = true;
-        }
+        if (tun->variant > 0) D->variant = tun->variant - 1;
         AF_REQUIRE(tun->tile_order == 0 || tun->tile_order == 1, AF_E_INVALID,
                    "tile_order %d (0 or 1)", tun->tile_order);
         D->tile_order = tun->tile_order;
@@ -794,9 +796,7 @@ int launch_step_v(af_domain* D, const TileDesc* tiles, int ntiles, int kmod6, hi
 // Kernel variants (af_tuning.variant - 1): rows per loop step (PF = 1: one
 // row, the next in flight; PF = 2: two-row load and store bursts) and the
 // occupancy (waves per SIMD) the register allocation targets and the launch
-// holds (occupancy_lds).  The one-row FP64 variant runs at three waves per
-// SIMD, which serves many short tiles (the ensembles); the two-row FP64
-// default at two, which serves the large domains.
+// holds (occupancy_lds).
 int launch_tiles(af_domain* D, const TileDesc* t, int n, int kmod6, hipStream_t st) {
     if (n == 0) return AF_OK;
     if (D->precision == AF_FP64) {
@@ -1154,22 +1154,6 @@ int retile_on_drain(af_domain* D, hipStream_t st) {
     return AF_OK;
 }
 
-// The default kernel variant (af_tuning.variant 0) at FP64, from the tile
-// count of the largest scenario: the two-row bursts at two waves per SIMD
-// stream the large domains best, the one-row loop at three waves the short
-// tile marches of small and mid-size ones.  Measured, two-row against
-// one-row (ms per step at the best tile height of each): roofline case at
-// 8192^2 0.813 against 0.868; valley at 4096^2 0.320 against 0.289; the
-// 512-scenario ensemble's autotuned best is the one-row variant (2.35 ms);
-// the saturated served pool 39.2 against 44.2 GCUPS.  At FP32 the two-row
-// variant stays the default everywhere (within 3% of the best).
-constexpr int kBurstVariantTiles = 8192;
-
-void choose_variant(af_domain* D, int most_tiles) {
-    if (D->variant_fixed || D->precision != AF_FP64) return;
-    D->variant = (most_tiles >= kBurstVariantTiles) ? 1 : 0;
-}
-
 int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn, int max_tiles,
               void* arena, size_t arena_bytes) {
     int rc = configure(D, precision, tuning);
@@ -1295,18 +1279,16 @@ int af_batch_create(const af_scenario_desc* descs, int n, int precision, const a
     af_domain tmp;
     (void)configure(&tmp, precision, tuning);
     tmp.min_rows = (n == 1) ? kMinRowsAlone : kMinRowsShared;
-    int tiles = 0, most = 0;
+    int tiles = 0;
     for (int s = 0; s < n; s++) {
         HostScn h;
         (void)scenario_bytes(&tmp, &descs[s], h);
         tiles += h.ntiles;
-        most = std::max(most, h.ntiles);
     }
     af_domain* D = new af_domain();
     hipStream_t st = to_stream(stream);
     rc = pool_init(D, precision, tuning, n, tiles, arena, arena_bytes);
     D->min_rows = tmp.min_rows;
-    if (rc == AF_OK) choose_variant(D, most);
     for (int s = 0; s < n && rc == AF_OK; s++) {
         int slot = -1;
         rc = pool_admit(D, &descs[s], st, false, &slot);
@@ -1354,7 +1336,6 @@ int af_pool_create(int precision, const af_tuning* tuning, int max_scenarios, in
     af_domain* D = new af_domain();
     int rc = pool_init(D, precision, tuning, max_scenarios, max_tiles, arena, arena_bytes);
     D->pool = true;
-    if (rc == AF_OK) choose_variant(D, 0);   // a pool holds many small scenarios
     if (rc) {
         delete D;
         return rc;
