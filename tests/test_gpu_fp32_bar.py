@@ -3,11 +3,13 @@ with 80 m relief, NODATA blobs, 5000 QFIX sources, free edges), FP32 and an
 FP64 twin (every terrain value moved by one ulp) against FP64, in lockstep.
 
 The flow amplifies any difference: the FP64 twin, a perturbation at FP64's
-own rounding level, leaves 1e-9 m within a few hundred steps and ends metres
-away, as FP32 does.  FP32 is therefore held (a) to the plain bar, 1e-3 m and
-1e-5 relative volume, over the first 50 steps, and (b) at every checkpoint to
-what its precision allows on this flow: its error at most 8 * 2^29 times the
-twin's (2^29 = FP32 unit roundoff / FP64 unit roundoff), or the bar."""
+own rounding level, stays within 1e-9 m through step 200 and then leaves it,
+ending metres away, as FP32 does.  FP32 is therefore held (a) to the plain
+bar, 1e-3 m and 1e-5 relative volume, over the first 100 steps (measured
+2.9e-4 m and 1.4e-7 at step 100; 1.4e-3 m at step 200, where the twin is at
+3.5e-11 m), and (b) at every checkpoint to what its precision allows on this
+flow: its error at most 8 * 2^29 times the twin's (2^29 = FP32 unit roundoff /
+FP64 unit roundoff), or the bar."""
 import numpy as np
 import pytest
 
@@ -47,7 +49,7 @@ def test_fp32_bar_on_the_roofline_generator(gpu):
         err, mass = _err(c.read("H"), Ha, dA)
         print(f"roofline 2048^2 step {at}: fp32 {err:.2e} / {mass:.2e}; fp64 twin {terr:.2e} / "
               f"{tmass:.2e}", flush=True)
-        if at <= 50:
+        if at <= 100:
             assert err <= 1e-3 and mass <= 1e-5, at
         assert err <= max(1e-3, K_SCALED * EPS_RATIO * terr), at
         assert mass <= max(1e-5, K_SCALED * EPS_RATIO * tmass), at
