@@ -1179,12 +1179,18 @@ int pool_init(af_domain* D, int precision, const af_tuning* tuning, int max_scn,
     D->d_ctlptr = reinterpret_cast<Ctl**>(D->arena + D->o_ctlptr);
     D->d_ctlbuf = reinterpret_cast<Ctl*>(D->arena + D->o_ctlbuf);
     D->active.clear();
-    // LDS limits of the device the domain lives on (the occupancy cap below)
+    // LDS limits of the device the domain lives on, for the occupancy cap of
+    // occupancy_lds; without them the launches reserve nothing
     int dev = 0;
-    AF_HIP(hipGetDevice(&dev));
-    AF_HIP(hipDeviceGetAttribute(&D->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
-                                 dev));
-    AF_HIP(hipDeviceGetAttribute(&D->lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    D->lds_cu = D->lds_block = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&D->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              dev) != hipSuccess ||
+        hipDeviceGetAttribute(&D->lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess) {
+        D->lds_cu = D->lds_block = 0;
+        (void)hipGetLastError();   // the failed query is not the next launch's error
+    }
     return AF_OK;
 }
 
