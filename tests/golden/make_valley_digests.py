@@ -17,13 +17,23 @@ and a later run with the same DIR resumes from it.  With --emit, the fixture
 is written from the checkpoints in DIR so far (no stepping): its "final" is
 null until the run has ended, and "dt_sha256" covers the first "dt_steps"
 entries of the dt log.
+
+To continue a run on another machine: --pack DIR PACKDIR writes the latest
+checkpoint of DIR in a compact form (H, Qx and Qy byte-shuffled and
+deflated, about half the size; Qxold and Qyold are Qx and Qy times 1/dx,
+as UpdateQs_GPU stores them, and are rebuilt); on the other machine
+--from-pack PACKDIR --checkpoint DIR unpacks it into DIR and resumes.
+--seconds S stops stepping after about S seconds and --out FILE writes the
+(partial) fixture to FILE after every checkpoint.
 """
 import argparse
 import hashlib
 import json
 import os
+import shutil
 import sys
 import time
+import zlib
 
 import numpy as np
 
@@ -102,38 +112,96 @@ def write_fixture(scn, checkpoints, final, dts, reports, seconds):
           + ("" if final else " (partial)"))
 
 
+PACKED = ("H", "Qx", "Qy")
+
+
+def pack(src, dst):
+    """The latest checkpoint of `src` in compact form, in `dst`."""
+    os.makedirs(dst, exist_ok=True)
+    with np.load(os.path.join(src, "state.npz"), allow_pickle=False) as z:
+        for k in PACKED:
+            a = np.ascontiguousarray(z[k], dtype="<f8")
+            shuffled = a.view(np.uint8).reshape(-1, 8).T.tobytes()
+            with open(os.path.join(dst, k + ".zsh"), "wb") as f:
+                f.write(zlib.compress(shuffled, 6))
+        np.savez(os.path.join(dst, "small.npz"), scalars=z["scalars"], dts=z["dts"],
+                 reports=z["reports"], sizes=np.array([z["H"].size, z["Qx"].size]))
+    shutil.copy(os.path.join(src, "checkpoints.json"), os.path.join(dst, "checkpoints.json"))
+
+
+def unpack(src, dst, dx):
+    """A packed checkpoint from `src` as the checkpoint of `dst`."""
+    os.makedirs(dst, exist_ok=True)
+    with np.load(os.path.join(src, "small.npz"), allow_pickle=False) as small:
+        sizes = dict(zip(PACKED, (int(small["sizes"][0]), int(small["sizes"][1]),
+                                  int(small["sizes"][1]))))
+        arr = {}
+        for k in PACKED:
+            with open(os.path.join(src, k + ".zsh"), "rb") as f:
+                raw = np.frombuffer(zlib.decompress(f.read()), dtype=np.uint8)
+            arr[k] = np.ascontiguousarray(raw.reshape(8, sizes[k]).T).view("<f8").ravel()
+        dxinv = 1.0 / dx    # UpdateQs_GPU: q_old = Q * (1.0 / dx)
+        tmp = os.path.join(dst, "state.tmp.npz")
+        np.savez(tmp, H=arr["H"], Qx=arr["Qx"], Qy=arr["Qy"], Qxold=arr["Qx"] * dxinv,
+                 Qyold=arr["Qy"] * dxinv, scalars=small["scalars"], dts=small["dts"],
+                 reports=small["reports"])
+    os.replace(tmp, os.path.join(dst, "state.npz"))
+    shutil.copy(os.path.join(src, "checkpoints.json"), os.path.join(dst, "checkpoints.json"))
+
+
+def record(r):
+    st = r.state()
+    return st, {"steps": st["steps"], "t": st["t"], "H": sha(st["H"]), "Qx": sha(st["Qx"]),
+                "Qy": sha(st["Qy"]), "H_max": float(st["H"].max()),
+                "H_sum": float(st["H"].sum())}
+
+
 def main():
+    global OUT
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--checkpoint", default=None)
     ap.add_argument("--emit", action="store_true")
+    ap.add_argument("--pack", nargs=2, metavar=("DIR", "PACKDIR"), default=None)
+    ap.add_argument("--from-pack", default=None)
+    ap.add_argument("--seconds", type=float, default=0.0)
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
+    if a.out:
+        OUT = os.path.abspath(a.out)
+    if a.pack:
+        return pack(*a.pack)
     if a.emit:
         return emit(a.checkpoint)
     if a.threads:
         oracle.set_threads(a.threads)
     scn = generators.valley_case(n=4096)
+    if a.from_pack:
+        unpack(a.from_pack, a.checkpoint, scn.dx)
     r = oracle.OracleRun(scn, dt_log_cap=1 << 18, mass_log_cap=1024)
     checkpoints = load_state(a.checkpoint, r) if a.checkpoint else []
     if checkpoints:
         print(f"resumed at step {r.steps}, t={r.t:.3f}", flush=True)
     t0 = time.time()
-    while r.t < scn.sim_time:
-        n = r.advance(CHECK)
-        st = r.state()
-        rec = {"steps": st["steps"], "t": st["t"], "H": sha(st["H"]), "Qx": sha(st["Qx"]),
-               "Qy": sha(st["Qy"]), "H_max": float(st["H"].max()),
-               "H_sum": float(st["H"].sum())}
+    final = None
+    # pieces of at most 500 steps, so that --seconds is kept to within a piece
+    while r.t < scn.sim_time and not (a.seconds and time.time() - t0 > a.seconds):
+        nxt = (r.steps // CHECK + 1) * CHECK
+        r.advance(min(500, nxt - r.steps))
+        if r.steps < nxt and r.t < scn.sim_time:
+            continue
+        st, rec = record(r)
         print(f"{time.time() - t0:8.0f} s  steps={st['steps']} t={st['t']:.3f} "
               f"H_max={rec['H_max']:.6f}", flush=True)
-        if n == CHECK and r.t < scn.sim_time:
+        if r.t < scn.sim_time:
             checkpoints.append(rec)
             if a.checkpoint:
                 save_state(a.checkpoint, r, checkpoints)
         else:
             final = rec
-    st = r.state()
-    write_fixture(scn, checkpoints, final, st["dt"], st["reports"], time.time() - t0)
+        if a.out or final:
+            write_fixture(scn, checkpoints, final, st["dt"], st["reports"],
+                          time.time() - t0 if final else None)
 
 
 if __name__ == "__main__":
