@@ -1,7 +1,13 @@
 import os
 import sys
 
-import pytest
+# The oracle's OpenMP threads sleep between parallel regions instead of
+# spinning: the many short regions of small test grids otherwise cost far
+# more than their work whenever another process holds the cores.  (Set before
+# anything loads an OpenMP runtime.)
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
+
+import pytest  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
