@@ -132,6 +132,7 @@ SIGNATURES = [
     ("af_pool_advance_lane", _I, [_V, _I64, _I, _V]),
     ("af_pool_lane_ms", _I, [_V, C.POINTER(C.c_float)]),
     ("af_pool_stats", _I, [_V, _ip, _ip, C.POINTER(C.c_size_t), C.POINTER(_I64)]),
+    ("af_domain_variant", _I, [_V, _ip]),
     ("af_par_defaults", None, [C.POINTER(Par)]),
     ("af_read_par", _I, [C.c_char_p, C.POINTER(Par)]),
     ("af_case_new", _V, []),

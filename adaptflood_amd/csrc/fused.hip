@@ -42,6 +42,7 @@ struct HostScn {
     int R_alone = 0;             // tile rows it would take running alone (<= R)
     int cap_tiles = 0;           // tiles at R_alone: the per-tile arrays are sized for it
     double zbase = 0.0, zscale = 1.0;   // fixed-point terrain of the FP32 path
+    long long wet0 = 0;          // cells deeper than 0.01 m in the initial state
     // offsets in the arena
     size_t o_z, o_H[2], o_Qx[2], o_Qy[2];
     size_t o_src_begin, o_src_j, o_src_i, o_src_val, o_src_off, o_src_len;
@@ -136,6 +137,7 @@ struct Staged {
     HostBuf z, h;   // P*ny elements each: FP64 elevations and depths, or the
                     // FP32 path's fixed-point terrain (int) and float depths
     double zbase = 0.0, zscale = 1.0, hmax = -DBL_MAX;
+    long long wet = 0;   // cells deeper than 0.01 m (the Area criterion of main.cpp:296)
 };
 
 }  // namespace
@@ -193,6 +195,10 @@ struct af_domain {
     bool lane_pending = false;        // lane work not yet joined into the caller's stream
     bool lane_timed = false;          // ev_t0 / ev_join hold a lane burst
     int lds_cu = 0, lds_block = 0;    // the device's LDS bytes per CU and per workgroup
+    // FP64 with af_tuning.variant 0: the kernel variant follows the wet area
+    // (adapt_variant); most_tiles is the largest scenario's tile count
+    bool variant_auto = false;
+    int most_tiles = 0;
 };
 
 namespace af {
@@ -432,15 +438,11 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
     D->V = precision == AF_FP64 ? 2 : 4;
     D->W = 64 * D->V;
     // Defaults measured on the 16384 x 16384 roofline case: 64-row tiles with
-    // two-row bursts at FP32; 32-row tiles with the one-row loop at three
-    // waves per SIMD at FP64.  The FP64 two-row variant at two waves is
-    // faster while the flow is dry (3.10 against 3.30 ms per step over the
-    // first 110 steps) and slower once it wets (the exact pow(h, 10/3) runs
-    // in every wavefront with a wet face): over the full 3600 s run 3.55
-    // against 3.20 ms per step.  The one-row FP64 variant is also the faster
-    // on the 4096^2 valley, the ragged ensemble and the served pool.
+    // two-row bursts at FP32, 32-row tiles at FP64, where the variant follows
+    // the wet area (adapt_variant).
     D->R = precision == AF_FP64 ? 32 : 64;
     D->variant = precision == AF_FP64 ? 0 : 1;
+    D->variant_auto = (precision == AF_FP64);
     D->wpb = 4;
     D->graph_steps = 36;
     D->dt_log_cap = 0;
@@ -454,6 +456,7 @@ int configure(af_domain* D, int precision, const af_tuning* tun) {
         AF_REQUIRE(tun->variant >= 0 && tun->variant <= 3, AF_E_INVALID,
                    "variant %d (0 = default, 1..3)", tun->variant);
         if (tun->variant > 0) D->variant = tun->variant - 1;
+        D->variant_auto = (precision == AF_FP64 && tun->variant == 0);
         AF_REQUIRE(tun->tile_order == 0 || tun->tile_order == 1, AF_E_INVALID,
                    "tile_order %d (0 or 1)", tun->tile_order);
         D->tile_order = tun->tile_order;
@@ -556,7 +559,7 @@ void terrain_scale(const af_scenario_desc* d, double& zbase, double& zscale) {
 // fluxes to NaN on the first step.
 template <typename T, typename Z>
 int convert_inputs(const af_scenario_desc* d, long long P, Z* z, T* H, double& zbase,
-                   double& zscale, double& hmax) {
+                   double& zscale, double& hmax, long long& wet) {
     constexpr bool f64 = sizeof(T) == 8;
     zbase = 0.0;
     zscale = 1.0;
@@ -564,6 +567,7 @@ int convert_inputs(const af_scenario_desc* d, long long P, Z* z, T* H, double& z
     memset(z, 0, sizeof(Z) * (size_t)P * d->ny);
     memset(H, 0, sizeof(T) * (size_t)P * d->ny);
     hmax = -DBL_MAX;
+    wet = 0;
     for (int j = 0; j < d->ny; j++) {
         for (int i = 0; i < d->nx; i++) {
             const long long c = i + (long long)j * d->nx;
@@ -581,6 +585,7 @@ int convert_inputs(const af_scenario_desc* d, long long P, Z* z, T* H, double& z
                        d->h0[c]);
             H[(size_t)j * P + i] = ht;
             if (!nod && (double)ht > hmax) hmax = (double)ht;  // kernels.cpp:32
+            if ((double)ht > 0.01) wet++;
         }
     }
     return AF_OK;
@@ -600,9 +605,9 @@ int stage_inputs(const af_scenario_desc* d, int precision, Staged& s) {
                "stage: no host memory for %zu bytes", 2 * bytes);
     if (precision == AF_FP64)
         return convert_inputs<double, double>(d, s.P, (double*)s.z.data(), (double*)s.h.data(),
-                                              s.zbase, s.zscale, s.hmax);
+                                              s.zbase, s.zscale, s.hmax, s.wet);
     return convert_inputs<float, int>(d, s.P, (int*)s.z.data(), (float*)s.h.data(), s.zbase,
-                                      s.zscale, s.hmax);
+                                      s.zscale, s.hmax, s.wet);
 }
 
 // Zeroes `words` 4-byte words at p (16-byte aligned): 16-byte stores for the
@@ -709,6 +714,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     int zrc = zero_fill(A + h.o_H[0], h.o_Qy[1] + faces_y * es - h.o_H[0], st);
     if (zrc) return zrc;
     const double hmax = pre.hmax;
+    h.wet0 = pre.wet;
     h.zbase = pre.zbase;
     h.zscale = pre.zscale;
     AF_HIP(hipMemcpyAsync(A + h.o_z, pre.z.data(), cells * es, hipMemcpyHostToDevice, st));
@@ -753,6 +759,7 @@ int upload_scenario(af_domain* D, int s, const af_scenario_desc* d, hipStream_t 
     c.t[cur ^ 1] = 0.0;
     c.next_mass[cur ^ 1] = d->mass_int;
     c.par[0] = c.par[1] = parity;  // the initial state sits in buffer `parity`
+    c.last_wet = (unsigned long long)h.wet0;
     put(h.o_ctl, &c, sizeof(c));
     AF_HIP(hipMemcpyAsync(A + aux_lo, aux.data(), aux.size(), hipMemcpyHostToDevice, st));
     // the host sources (aux, the staged inputs) must outlive the copies
@@ -934,6 +941,34 @@ __global__ void k_gather_ctl(Ctl* const* __restrict__ list, int n, Ctl* __restri
 // read with one gather launch, one copy and one synchronisation.
 int settle(af_domain* D, hipStream_t st);
 
+// The FP64 default variant follows the flow.  The exact pow(h, 10/3) runs in
+// every wavefront with a wet face, and more waves per SIMD hide its chain:
+// the one-row loop at three waves (variant 1) is the faster on a wet flow and
+// on small and mid-size domains; the two-row bursts at two waves (variant 2)
+// only on a large, nearly dry one.  Measured on the 16384^2 roofline case
+// (ms per step): over its first 110 steps 3.10 (variant 2) against 3.30;
+// over the full 3600 s run 3.55 against 3.20, variant 1 ahead from between
+// t = 300 and 500 s, when 2.5 to 3% of the cells are wet.  Variant 1 is also the
+// faster on the 4096^2 valley (0.288 against 0.320), the ragged ensemble
+// and the served pool; variant 2 on a dry 8192^2 roofline case (0.813
+// against 0.868).  So variant 2 is used while the largest scenario has at
+// least kBurstTiles tiles and under kBurstWet of all cells are wet (a pool
+// uses variant 1).  The wet count is the last mass report's (or the initial
+// state's); the choice is made wherever the control blocks are read back,
+// and a change drops the captured graphs, which the next advance captures
+// again.  The variants compute bitwise the same results.
+constexpr int kBurstTiles = 8192;
+constexpr double kBurstWet = 0.03;
+
+void adapt_variant(af_domain* D, double wet, double cells) {
+    if (!D->variant_auto) return;
+    const int want = (D->most_tiles >= kBurstTiles && wet < kBurstWet * cells) ? 1 : 0;
+    if (want == D->variant) return;
+    D->variant = want;
+    drop_graph(D->graph, D->gexec);
+    drop_graph(D->lgraph, D->lgexec);
+}
+
 int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
     int rc = settle(D, st);
     if (rc) return rc;
@@ -946,6 +981,12 @@ int gather_ctl(af_domain* D, std::vector<Ctl>& out, hipStream_t st) {
     AF_HIP(hipGetLastError());
     AF_HIP(hipMemcpyAsync(out.data(), D->d_ctlbuf, sizeof(Ctl) * n, hipMemcpyDeviceToHost, st));
     AF_HIP(hipStreamSynchronize(st));
+    double wet = 0.0, cells = 0.0;
+    for (int a = 0; a < n; a++) {
+        wet += (double)out[a].last_wet;
+        cells += (double)D->scn[D->active[a]].nx * D->scn[D->active[a]].ny;
+    }
+    adapt_variant(D, wet, cells);
     return AF_OK;
 }
 
@@ -1285,20 +1326,28 @@ int af_batch_create(const af_scenario_desc* descs, int n, int precision, const a
     af_domain tmp;
     (void)configure(&tmp, precision, tuning);
     tmp.min_rows = (n == 1) ? kMinRowsAlone : kMinRowsShared;
-    int tiles = 0;
+    int tiles = 0, most = 0;
     for (int s = 0; s < n; s++) {
         HostScn h;
         (void)scenario_bytes(&tmp, &descs[s], h);
         tiles += h.ntiles;
+        most = std::max(most, h.ntiles);
     }
     af_domain* D = new af_domain();
     hipStream_t st = to_stream(stream);
     rc = pool_init(D, precision, tuning, n, tiles, arena, arena_bytes);
     D->min_rows = tmp.min_rows;
+    double wet = 0.0, cells = 0.0;
     for (int s = 0; s < n && rc == AF_OK; s++) {
         int slot = -1;
         rc = pool_admit(D, &descs[s], st, false, &slot);
+        if (rc == AF_OK) {
+            wet += (double)D->scn[slot].wet0;
+            cells += (double)descs[s].nx * descs[s].ny;
+        }
     }
+    D->most_tiles = most;
+    if (rc == AF_OK) adapt_variant(D, wet, cells);
     if (rc == AF_OK) rc = rebuild_tiles(D, st);
     if (rc) {
         delete D;
@@ -1454,6 +1503,12 @@ int af_pool_stats(af_domain* D, int32_t* active, int32_t* tiles, size_t* heap_fr
     if (tiles) *tiles = nt;
     if (heap_free) *heap_free = heap_free_bytes(D);
     if (step_index) *step_index = D->k;
+    return AF_OK;
+}
+
+int af_domain_variant(af_domain* D, int32_t* variant) {
+    AF_REQUIRE(D && variant, AF_E_INVALID, "variant: bad arguments");
+    *variant = D->variant + 1;
     return AF_OK;
 }
 

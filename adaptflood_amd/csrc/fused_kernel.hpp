@@ -46,6 +46,9 @@ struct Ctl {
     // another wave of the same step set: every wave of a scenario takes the
     // same skip decision.
     int status[2];
+    // cells deeper than 0.01 m at the last finalized mass report (the
+    // initial state's before the first), for the host's choice of variant
+    unsigned long long last_wet;
 };
 
 // Ctl.status bits (the values of AF_SCN_NONFINITE / AF_SCN_STALLED).
@@ -876,6 +879,7 @@ struct Stepper {
                 rec[3] = (double)wet * S.dA;
             }
             ctl->n_reports = r + 1;
+            ctl->last_wet = wet;
             ctl->pending = 0;
         }
     }

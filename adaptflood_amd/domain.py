@@ -101,6 +101,13 @@ class Batch:
                 "n_reports": r.n_reports, "finished": bool(r.finished), "hmax": r.hmax,
                 "status": r.status}
 
+    def variant(self):
+        """The kernel variant the next launches use (1..3, af_tuning.variant's
+        numbering; at FP64 with variant 0 it follows the wet area)."""
+        v = C.c_int32()
+        call("af_domain_variant", self.handle, C.byref(v))
+        return int(v.value)
+
     def set_sim_time(self, sim_time, i=0):
         """Moves the stop time of scenario i (runs stop at the first step with
         t >= sim_time, as the reference's loop does)."""

@@ -375,6 +375,15 @@ int af_pool_lane_ms(af_domain* pool, float* ms);
 int af_pool_stats(af_domain* pool, int32_t* active, int32_t* tiles, size_t* heap_free,
                   int64_t* step_index);
 
+/* The kernel variant the next launches use (1..3, af_tuning.variant's
+ * numbering).  With af_tuning.variant 0 at FP64 it follows the flow: the
+ * two-row variant (2) while the largest scenario has 8192 tiles or more and
+ * under 3% of the cells are wet, the one-row variant (1) otherwise; the wet
+ * count (the last mass report's) is re-read whenever every control block is
+ * read back (af_domain_run, af_batch_finished, af_pool_poll).  No counterpart
+ * in the reference (it has one kernel sequence). */
+int af_domain_variant(af_domain* d, int32_t* variant);
+
 /* ---------------------------------------------------------------------- */
 /* Input formats (host): .par, .dem.ascii, .bci, .bdy, rain                */
 /* ---------------------------------------------------------------------- */

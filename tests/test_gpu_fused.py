@@ -405,3 +405,32 @@ def test_many_scenarios_poll_and_run_to_their_own_ends(gpu):
         if len(reported) == 48:
             break
     assert len(reported) == 48
+
+
+def test_fp64_variant_follows_the_wet_area(gpu):
+    """FP64 with the default tuning: a large domain (8192 tiles of 128 x 32
+    cells) starts on the two-row variant while it is dry and moves to the
+    one-row variant once a mass report counts more than 2% of its cells wet
+    (here after the first step of heavy rain).  The same run with either
+    variant fixed gives bitwise the same states, steps and reports."""
+    from adaptflood_amd.scenario import Scenario
+    ny, nx = 2048, 16384
+    rng = np.random.default_rng(21)
+    dem = np.linspace(8.0, 0.0, nx)[None, :] + rng.uniform(0.0, 0.5, (ny, nx))
+    scn = Scenario(dem=dem, dx=5.0, sim_time=40.0, mass_int=5.0, east_free=True,
+                   rain_times=np.array([0.0, 1e4]), rain_rates=np.array([2e-3, 2e-3]),
+                   name="rain on a tilted plane, 16384 x 2048")
+    d = Domain(scn, precision="fp64")
+    assert d.variant() == 2
+    d.run(check_every=6, graph=True)
+    assert d.variant() == 1
+    H, info, rep = d.read("H"), d.info(), d.reports()
+    d.close()
+    for v in (1, 2):
+        f = Domain(scn, precision="fp64", tuning=make_tuning(variant=v))
+        f.run(check_every=6, graph=True)
+        assert f.variant() == v
+        assert f.info()["steps"] == info["steps"] and f.info()["t"] == info["t"]
+        assert np.array_equal(f.read("H"), H), v
+        assert np.array_equal(f.reports(), rep), v
+        f.close()
