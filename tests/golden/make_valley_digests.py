@@ -23,8 +23,16 @@ checkpoint of DIR in a compact form (H, Qx and Qy byte-shuffled and
 deflated, about half the size; Qxold and Qyold are Qx and Qy times 1/dx,
 as UpdateQs_GPU stores them, and are rebuilt); on the other machine
 --from-pack PACKDIR --checkpoint DIR unpacks it into DIR and resumes.
---seconds S stops stepping after about S seconds and --out FILE writes the
-(partial) fixture to FILE after every checkpoint.
+--seconds S stops stepping after about S seconds (recording the state it
+stopped at as a checkpoint) and --out FILE writes the (partial) fixture to
+FILE after every checkpoint.
+
+--from-fixture FILE resumes from the last checkpoint of a partial fixture
+without a saved state (the state of a run is too large to carry between
+machines): the FP64 fused path (GPU) is advanced to that step, and its H, Qx,
+Qy and dt log are taken as the oracle's state only if their SHA-256 digests
+equal the oracle's own digests of that step; the mass reports and the clock
+come from the fixture.  Every later digest is the oracle's own stepping.
 """
 import argparse
 import hashlib
@@ -149,6 +157,46 @@ def unpack(src, dst, dx):
     shutil.copy(os.path.join(src, "checkpoints.json"), os.path.join(dst, "checkpoints.json"))
 
 
+def resume(r, fx, H, Qx, Qy, dts):
+    """Sets the oracle run `r` to the last checkpoint of the partial fixture
+    `fx`, given a state (H, Qx, Qy, dt log) whose digests must equal that
+    checkpoint's; returns the fixture's checkpoints."""
+    rec = fx["checkpoints"][-1]
+    if fx["final"] is not None or fx["dt_steps"] != rec["steps"] or len(dts) != rec["steps"]:
+        raise ValueError("the fixture's last checkpoint is not a resumable partial run")
+    for name, a in (("H", H), ("Qx", Qx), ("Qy", Qy)):
+        if sha(a) != rec[name]:
+            raise ValueError(f"{name} at step {rec['steps']} differs from the oracle's")
+    if sha(dts) != fx["dt_sha256"]:
+        raise ValueError(f"dt log to step {rec['steps']} differs from the oracle's")
+    r.H[:] = np.asarray(H, dtype=np.float64).ravel()
+    r.Qx[:] = Qx
+    r.Qy[:] = Qy
+    dxinv = 1.0 / r.s.dx    # UpdateQs_GPU: q_old = Q * (1.0 / dx)
+    r.Qxold[:] = r.Qx * dxinv
+    r.Qyold[:] = r.Qy * dxinv
+    r.t, r.steps = rec["t"], rec["steps"]
+    r.dts = np.asarray(dts, dtype=np.float64).tolist()
+    r.reports = [list(map(float, x)) for x in fx["reports"]]
+    nm = float(r.s.mass_int)   # one report per crossing: next_mass += mass_int
+    for _ in r.reports:
+        nm += r.s.mass_int
+    r.next_mass = nm
+    return list(fx["checkpoints"])
+
+
+def fused_state(scn, steps):
+    """H, Qx, Qy and the dt log of the FP64 fused path after `steps` steps."""
+    from adaptflood_amd.domain import Domain, make_tuning
+    d = Domain(scn, precision="fp64", tuning=make_tuning(dt_log_cap=steps + 64, report_cap=1024))
+    d.advance(steps, graph=True)
+    if d.info()["steps"] != steps:
+        raise RuntimeError("the fused path stopped early")
+    out = d.read("H"), d.read("Qx"), d.read("Qy"), d.dt_log()[:steps]
+    d.close()
+    return out
+
+
 def record(r):
     st = r.state()
     return st, {"steps": st["steps"], "t": st["t"], "H": sha(st["H"]), "Qx": sha(st["Qx"]),
@@ -166,6 +214,7 @@ def main():
     ap.add_argument("--from-pack", default=None)
     ap.add_argument("--seconds", type=float, default=0.0)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--from-fixture", default=None)
     a = ap.parse_args()
     if a.out:
         OUT = os.path.abspath(a.out)
@@ -180,16 +229,28 @@ def main():
         unpack(a.from_pack, a.checkpoint, scn.dx)
     r = oracle.OracleRun(scn, dt_log_cap=1 << 18, mass_log_cap=1024)
     checkpoints = load_state(a.checkpoint, r) if a.checkpoint else []
+    if a.from_fixture:
+        with open(a.from_fixture) as f:
+            fx = json.load(f)
+        steps = fx["checkpoints"][-1]["steps"]
+        print(f"fused FP64 path to step {steps}", flush=True)
+        checkpoints = resume(r, fx, *fused_state(scn, steps))
+        print("its state and dt log equal the oracle's digests", flush=True)
     if checkpoints:
         print(f"resumed at step {r.steps}, t={r.t:.3f}", flush=True)
     t0 = time.time()
     final = None
     # pieces of at most 500 steps, so that --seconds is kept to within a piece
-    while r.t < scn.sim_time and not (a.seconds and time.time() - t0 > a.seconds):
-        nxt = (r.steps // CHECK + 1) * CHECK
-        r.advance(min(500, nxt - r.steps))
-        if r.steps < nxt and r.t < scn.sim_time:
-            continue
+    while r.t < scn.sim_time:
+        out_of_time = a.seconds and time.time() - t0 > a.seconds
+        if not out_of_time:
+            nxt = (r.steps // CHECK + 1) * CHECK
+            r.advance(min(500, nxt - r.steps))
+            if r.steps < nxt and r.t < scn.sim_time:
+                print(f"{time.time() - t0:8.0f} s  steps={r.steps} t={r.t:.3f}", flush=True)
+                continue
+        elif checkpoints and checkpoints[-1]["steps"] == r.steps:
+            break
         st, rec = record(r)
         print(f"{time.time() - t0:8.0f} s  steps={st['steps']} t={st['t']:.3f} "
               f"H_max={rec['H_max']:.6f}", flush=True)
@@ -202,7 +263,8 @@ def main():
         if a.out or final:
             write_fixture(scn, checkpoints, final, st["dt"], st["reports"],
                           time.time() - t0 if final else None)
-
+        if out_of_time:
+            break
 
 if __name__ == "__main__":
     main()
