@@ -984,6 +984,7 @@ def measure_wet(scn, args):
     wall = time.perf_counter() - t0
     kern = _event_ms(torch, d.launch_step, min(args.steps, 50)) * 1e-3
     info = d.info()
+    variant = d.variant()
     d.close()
     cells = scn.cells
     a64 = 56.0 * cells / kern / 1e9
@@ -994,6 +995,7 @@ def measure_wet(scn, args):
             "ms_per_step": round(wall * 1e3 / args.steps, 4),
             "state": f"after {args.wet_time:.0f} s of the run, {wet * 100:.1f}% of cells wet "
                      f"(H > 0.01 m)",
+            "variant": variant,
             "roofline": {"bound": "hbm", "achieved": round(a64, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(a64 / HBM_PEAK_GBS, 4),
                          "kernel_ms": round(kern * 1e3, 4)}}

@@ -2,11 +2,13 @@
 workload (fractal terrain with a carved valley at 5 m, 2000 QVAR hydrographs,
 20 mm/h rain over 0-1800 s, HFIX 2 m west edge, FREE N/E/S, 7200 s).
 
-The full run is about 76,000 steps, roughly an hour of the CPU oracle, so
-the fixture keeps SHA-256 digests of the state instead of the state itself:
+The full run is 76,456 steps, hours of the CPU oracle (about 0.09 s per step
+on 16 cores), so the fixture keeps SHA-256 digests of the state instead of
+the state itself:
 H (ny*nx), Qx and Qy ((ny+1)*(nx+1), pitch nx+1) as little-endian float64,
 at every CHECK steps and at the end, the digest of the whole dt log, the step
-count, the final t and every mass report (t, dt, Vol, Area).  A GPU test
+count, the final t and every mass report (t, dt, Vol, Area); its
+"oracle_seconds" is the stepping time of the run's last leg.  A GPU test
 advances the FP64 fused path by the same step counts and compares digests;
 the reports are compared as values (the volume sum's order differs).
 
