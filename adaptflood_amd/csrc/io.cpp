@@ -198,6 +198,48 @@ inline bool is_ws(char ch) {
     return ch == ' ' || ch == '\t' || ch == '\n' || ch == '\r' || ch == '\v' || ch == '\f';
 }
 
+// Exact powers of ten: 10^k is a double exactly for k <= 22.
+constexpr double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                               1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                               1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+
+// strtod for the plain decimal tokens a DEM holds ([+-]digits[.digits]
+// followed by whitespace or the end of the piece), when the digits form an
+// integer w <= 2^53 with at most 22 of them after the point: then w and
+// 10^f are doubles exactly and w / 10^f is one correctly rounded division,
+// which is the correctly rounded value of the token, as strtod returns it
+// (Clinger's fast path).  Returns false, leaving *x unset, for every other
+// token (exponents, more digits, nan/inf, hex, trailing text), which the
+// caller gives to strtod itself.
+inline bool parse_plain_decimal(const char* q, const char* e, double* x, const char** after) {
+    const char* p = q;
+    bool neg = false;
+    if (p < e && (*p == '+' || *p == '-')) neg = (*p++ == '-');
+    unsigned long long w = 0;
+    int sig = 0, frac = 0, digits = 0;
+    bool point = false;
+    for (; p < e; p++) {
+        const char ch = *p;
+        if (ch >= '0' && ch <= '9') {
+            digits++;
+            if (point) frac++;
+            if (w == 0 && ch == '0') continue;  // leading zeros add no significant digit
+            if (++sig > 16) return false;       // 10^16 > 2^53: beyond the exact range
+            w = w * 10 + (unsigned long long)(ch - '0');
+        } else if (ch == '.' && !point) {
+            point = true;
+        } else {
+            break;
+        }
+    }
+    if (digits == 0 || (p < e && !is_ws(*p))) return false;
+    if (w > (1ull << 53) || frac > 22) return false;
+    const double v = frac ? (double)w / kPow10[frac] : (double)w;
+    *x = neg ? -v : v;
+    *after = p;
+    return true;
+}
+
 // The value loop of read_dem (io.cpp:150-155): fscanf("%lf") per cell, in
 // order, until a token does not convert (every later fscanf fails on that
 // same token, so the remaining cells keep 0).  The text is cut at
@@ -230,8 +272,15 @@ void parse_values(const char* beg, const char* end, std::vector<double>& out) {
         while (true) {
             while (q < e && is_ws(*q)) q++;
             if (q >= e) break;
+            double x;
+            const char* fast = nullptr;
+            if (parse_plain_decimal(q, e, &x, &fast)) {
+                v.push_back(x);
+                q = fast;
+                continue;
+            }
             char* after = nullptr;
-            const double x = strtod(q, &after);
+            x = strtod(q, &after);
             if (after == q) {
                 stopped[t] = 1;
                 break;

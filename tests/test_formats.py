@@ -273,6 +273,42 @@ def test_large_dem_parallel_parse_and_stop_rule(tmp_path):
     np.testing.assert_array_equal(c.scenario().dem.ravel(), want)
 
 
+def test_dem_values_convert_as_strtod_does(tmp_path):
+    """The DEM reader's fast conversion of plain decimal tokens (and its
+    fallback to strtod for every other form) gives the correctly rounded
+    value strtod gives, bit for bit: Python's float() of a decimal string is
+    correctly rounded too.  Tokens: printf %.4f / %.6f / %.17g / %.3e of
+    random values over many magnitudes, 15 to 20 significant digits (either
+    side of the 2^53 limit), 22 and 23 digits after the point, leading
+    zeros, signs, '1.' and '.5' forms, -0.0, exponents and inf/nan."""
+    rng = np.random.default_rng(11)
+    vals = rng.uniform(-1, 1, 4000) * 10.0 ** rng.integers(-12, 12, 4000)
+    toks = []
+    for i, v in enumerate(vals):
+        toks.append(["%.4f", "%.6f", "%.17g", "%.3e"][i % 4] % v)
+    for k in range(14, 21):
+        for _ in range(40):
+            d = "".join(str(x) for x in rng.integers(0, 10, k))
+            cut = int(rng.integers(0, k + 1))
+            toks.append(d[:cut] + "." + d[cut:])
+    toks += ["9007199254740992", "9007199254740993", "9007199254740993.0", "0.9007199254740993",
+             "1." + "0" * 21 + "1", "1." + "0" * 22 + "1", "0." + "0" * 21 + "7",
+             "0000000000000000000000123.5", "-0.0", "-0", "+7.25", "1.", ".5", "-.5",
+             "1e3", "1E-3", "-2.5e+2", "inf", "-inf", "nan", "0.1", "0.3", "123456.7890",
+             "-9999", "-9999.0000", "1.7976931348623157e308", "4.9e-324"]
+    ny = 1
+    nx = len(toks)
+    want = np.array([float(t) for t in toks])
+    text = f"ncols {nx}\nnrows {ny}\nxllcorner 0\nyllcorner 0\ncellsize 5\nNODATA_value -9999\n"
+    p = _w(tmp_path, "tok.dem.ascii", text + " ".join(toks) + "\n")
+    c = formats.Case()
+    c.read_dem(p)
+    got = c.scenario().dem.ravel()
+    ok = (got.view(np.int64) == want.view(np.int64)) | (np.isnan(got) & np.isnan(want))
+    bad = [(toks[i], got[i], want[i]) for i in np.flatnonzero(~ok)[:5]]
+    assert ok.all(), bad
+
+
 def test_large_asc_write_matches_printf(tmp_path):
     """The threaded writer's bytes equal a sequential '%.4f ' per value, one
     line per row, under the reference header."""
