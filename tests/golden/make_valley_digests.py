@@ -13,7 +13,10 @@ advances the FP64 fused path by the same step counts and compares digests;
 the reports are compared as values (the volume sum's order differs).
 
 Run from the repository root:
-    python tests/golden/make_valley_digests.py [--threads N] [--checkpoint DIR]
+    python tests/golden/make_valley_digests.py [--case C] [--threads N] [--checkpoint DIR]
+--case roofline1024 runs the roofline generator at 1024 x 1024 (fractal
+terrain at 2 m, 10% NODATA, 5000 QFIX, free edges) for its full 3600 s
+instead, into tests/golden/roofline1024_fp64.json.
 With --checkpoint, the oracle state is saved to DIR after every CHECK steps
 and a later run with the same DIR resumes from it.  With --emit, the fixture
 is written from the checkpoints in DIR so far (no stepping): its "final" is
@@ -53,7 +56,17 @@ sys.path.insert(0, ROOT)
 from adaptflood_amd import generators  # noqa: E402
 from oracle import oracle  # noqa: E402
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "valley4096_fp64.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
+# --case: the workload and its fixture (tests/golden/<case>_fp64.json)
+CASES = {
+    "valley4096": (lambda: generators.valley_case(n=4096),
+                   "valley 4096x4096 at 5 m, 2000 QVAR, rain 0-1800 s, HFIX west, 7200 s"),
+    "roofline1024": (lambda: generators.roofline_case(n=1024),
+                     "roofline generator 1024x1024 at 2 m, fractal DEM, 10% NODATA, 5000 QFIX, "
+                     "FREE edges, 3600 s"),
+}
+CASE = "valley4096"
+OUT = os.path.join(HERE, f"{CASE}_fp64.json")
 CHECK = 5000
 
 
@@ -96,13 +109,13 @@ def emit(d):
         reports = z["reports"]
         steps = int(z["scalars"][2])
     assert checkpoints and checkpoints[-1]["steps"] == steps == len(dts)
-    scn = generators.valley_case(n=4096)
+    scn = CASES[CASE][0]()
     write_fixture(scn, checkpoints, None, dts, reports, None)
 
 
 def write_fixture(scn, checkpoints, final, dts, reports, seconds):
     out = {
-        "workload": "valley 4096x4096 at 5 m, 2000 QVAR, rain 0-1800 s, HFIX west, 7200 s",
+        "workload": CASES[CASE][1],
         "dem_sha256": hashlib.sha256(np.ascontiguousarray(scn.dem, dtype=np.float64)
                                      .tobytes()).hexdigest(),
         "check_every": CHECK,
@@ -207,8 +220,9 @@ def record(r):
 
 
 def main():
-    global OUT
+    global OUT, CASE
     ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="valley4096", choices=sorted(CASES))
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--checkpoint", default=None)
     ap.add_argument("--emit", action="store_true")
@@ -218,15 +232,15 @@ def main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--from-fixture", default=None)
     a = ap.parse_args()
-    if a.out:
-        OUT = os.path.abspath(a.out)
+    CASE = a.case
+    OUT = os.path.abspath(a.out) if a.out else os.path.join(HERE, f"{CASE}_fp64.json")
     if a.pack:
         return pack(*a.pack)
     if a.emit:
         return emit(a.checkpoint)
     if a.threads:
         oracle.set_threads(a.threads)
-    scn = generators.valley_case(n=4096)
+    scn = CASES[CASE][0]()
     if a.from_pack:
         unpack(a.from_pack, a.checkpoint, scn.dx)
     r = oracle.OracleRun(scn, dt_log_cap=1 << 18, mass_log_cap=1024)
