@@ -16,7 +16,8 @@ Run from the repository root:
     python tests/golden/make_valley_digests.py [--case C] [--threads N] [--checkpoint DIR]
 --case roofline1024 runs the roofline generator at 1024 x 1024 (fractal
 terrain at 2 m, 10% NODATA, 5000 QFIX, free edges) for its full 3600 s
-instead, into tests/golden/roofline1024_fp64.json.
+instead, into tests/golden/roofline1024_fp64.json; --case memberK one
+member of the ensemble generator (K in MEMBERS), into memberK_fp64.json.
 With --checkpoint, the oracle state is saved to DIR after every CHECK steps
 and a later run with the same DIR resumes from it.  With --emit, the fixture
 is written from the checkpoints in DIR so far (no stepping): its "final" is
@@ -65,6 +66,12 @@ CASES = {
                      "roofline generator 1024x1024 at 2 m, fractal DEM, 10% NODATA, 5000 QFIX, "
                      "FREE edges, 3600 s"),
 }
+# members of the 512-scenario ensemble generator (seed 4000 + k), run alone
+# for their full 3600 s; a GPU test steps them together in one batch
+MEMBERS = (3, 21, 31, 48, 69, 78, 151, 241, 292, 354, 374)
+for _k in MEMBERS:
+    CASES[f"member{_k}"] = (lambda k=_k: generators.ensemble_member(k),
+                            f"ensemble generator member {_k} (seed {4000 + _k}), 3600 s")
 CASE = "valley4096"
 OUT = os.path.join(HERE, f"{CASE}_fp64.json")
 CHECK = 5000

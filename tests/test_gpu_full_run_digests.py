@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from adaptflood_amd import generators
-from adaptflood_amd.domain import Domain, make_tuning
+from adaptflood_amd.domain import Batch, Domain, make_tuning
 
 pytestmark = pytest.mark.gpu
 
@@ -65,3 +65,58 @@ def test_fp64_matches_oracle_digests(gpu, case):
     assert np.array_equal(got[:, [0, 1, 3]], ref[:, [0, 1, 3]])
     n_eps = scn.cells * np.finfo(np.float64).eps / 2
     np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=n_eps, atol=0)
+
+
+# members of the ensemble generator with oracle digests of their full runs
+# (tests/golden/member<k>_fp64.json), from 8,112 to 3,205,940 cells
+MEMBERS = (3, 21, 31, 48, 69, 78, 151, 241, 292, 354, 374)
+
+
+def test_batched_members_match_oracle_digests(gpu):
+    """Eleven members of the ensemble generator stepped together in one batch
+    (one launch per step, each member with its own dt and clock, finished
+    members skipping) for their full 3600 s, against the oracle's runs of
+    each member alone: at every checkpoint of every member and at its end,
+    H, Qx and Qy equal the oracle's bitwise; so do its dt log, step count
+    and clock; report times, steps and areas are exact and volumes agree to
+    n*eps."""
+    fxs = []
+    for k in MEMBERS:
+        with open(os.path.join(GOLDEN, f"member{k}_fp64.json")) as f:
+            fxs.append(json.load(f))
+    scns = [generators.ensemble_member(k) for k in MEMBERS]
+    for k, s, fx in zip(MEMBERS, scns, fxs):
+        assert hashlib.sha256(np.ascontiguousarray(s.dem, dtype=np.float64).tobytes()
+                              ).hexdigest() == fx["dem_sha256"], k
+        assert fx["final"] is not None, k
+    cap = max(fx["dt_steps"] for fx in fxs) + 64
+    b = Batch(scns, precision="fp64", tuning=make_tuning(dt_log_cap=cap, report_cap=1024))
+    recs = [fx["checkpoints"] + [fx["final"]] for fx in fxs]
+    marks = sorted({r["steps"] for rs in recs for r in rs})
+    done = 0
+    for m in marks:
+        b.advance(m - done, graph=True)
+        done = m
+        for i, k in enumerate(MEMBERS):
+            for rec in recs[i]:
+                if rec["steps"] != m:
+                    continue
+                info = b.info(i)
+                print(f"member {k} step {info['steps']} t={info['t']:.6f} "
+                      f"(oracle {rec['t']:.6f})", flush=True)
+                assert info["steps"] == rec["steps"] and info["t"] == rec["t"], k
+                assert _sha(b.read("H", i)) == rec["H"], (k, m)
+                assert _sha(b.read("Qx", i)) == rec["Qx"], (k, m)
+                assert _sha(b.read("Qy", i)) == rec["Qy"], (k, m)
+    b.advance(12, graph=True)   # every member has finished: these launches skip
+    for i, (k, s, fx) in enumerate(zip(MEMBERS, scns, fxs)):
+        info = b.info(i)
+        assert info["finished"] and info["steps"] == fx["final"]["steps"], k
+        assert info["t"] == fx["final"]["t"], k
+        assert _sha(b.dt_log(i)[: fx["dt_steps"]]) == fx["dt_sha256"], k
+        ref = np.array(fx["reports"]).reshape(-1, 4)
+        got = b.reports(i)[: len(ref)]
+        assert len(got) == len(ref), k
+        assert np.array_equal(got[:, [0, 1, 3]], ref[:, [0, 1, 3]]), k
+        n_eps = s.cells * np.finfo(np.float64).eps / 2
+        np.testing.assert_allclose(got[:, 2], ref[:, 2], rtol=max(n_eps, 1e-15), atol=0)
