@@ -106,3 +106,40 @@ def test_libm_pow_leaves_the_bar_on_the_roofline_generator():
     assert 0.0 < gap[100] <= 1e-12
     assert gap[400] > 1e-9
     assert gap[600] > 1e-3
+
+
+def test_full_run_fixtures_are_consistent():
+    """The digest fixtures of the oracle's full runs (tests/golden/*_fp64.json,
+    checked against the FP64 fused path by tests/test_gpu_full_run_digests.py):
+    each names the DEM its generator still makes, its checkpoints climb to a
+    final record at t >= sim_time whose step count is the dt log's length,
+    and its mass reports come once per mass interval in time order."""
+    import glob
+    import hashlib
+    import json
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_valley_digests as mk
+    paths = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                          "golden", "*_fp64.json")))
+    cases = {os.path.basename(p)[: -len("_fp64.json")] for p in paths}
+    assert cases == set(mk.CASES), cases ^ set(mk.CASES)
+    for p in paths:
+        case = os.path.basename(p)[: -len("_fp64.json")]
+        with open(p) as f:
+            fx = json.load(f)
+        scn = mk.CASES[case][0]()
+        assert fx["workload"] == mk.CASES[case][1], case
+        assert hashlib.sha256(np.ascontiguousarray(scn.dem, dtype=np.float64).tobytes()
+                              ).hexdigest() == fx["dem_sha256"], case
+        recs = fx["checkpoints"] + [fx["final"]]
+        steps = [r["steps"] for r in recs]
+        ts = [r["t"] for r in recs]
+        assert steps == sorted(set(steps)) and ts == sorted(ts), case
+        assert fx["final"]["t"] >= scn.sim_time > (ts[-2] if len(ts) > 1 else 0.0), case
+        assert fx["dt_steps"] == fx["final"]["steps"], case
+        rep = np.array(fx["reports"]).reshape(-1, 4)
+        assert len(rep) == int(fx["final"]["t"] // scn.mass_int), case
+        assert np.all(np.diff(rep[:, 0]) > 0) and np.all(rep[:, 1] > 0), case
