@@ -1,10 +1,12 @@
-"""Writes the digests of the FP64 oracle's full run of the 4096 x 4096 valley
-workload (fractal terrain with a carved valley at 5 m, 2000 QVAR hydrographs,
-20 mm/h rain over 0-1800 s, HFIX 2 m west edge, FREE N/E/S, 7200 s).
+"""Writes the digests of the FP64 oracle's full run of one workload (--case):
+the 4096 x 4096 valley workload (fractal terrain with a carved valley at
+5 m, 2000 QVAR hydrographs, 20 mm/h rain over 0-1800 s, HFIX 2 m west edge,
+FREE N/E/S, 7200 s; the default), the roofline generator at 1024 x 1024, or
+a member of the ensemble generator (CASES below).
 
-The full run is 76,456 steps, hours of the CPU oracle (about 0.09 s per step
-on 16 cores), so the fixture keeps SHA-256 digests of the state instead of
-the state itself:
+The valley's full run is 76,456 steps, hours of the CPU oracle (about
+0.09 s per step on 16 cores), so a fixture keeps SHA-256 digests of the
+state instead of the state itself:
 H (ny*nx), Qx and Qy ((ny+1)*(nx+1), pitch nx+1) as little-endian float64,
 at every CHECK steps and at the end, the digest of the whole dt log, the step
 count, the final t and every mass report (t, dt, Vol, Area); its
@@ -13,7 +15,7 @@ advances the FP64 fused path by the same step counts and compares digests;
 the reports are compared as values (the volume sum's order differs).
 
 Run from the repository root:
-    python tests/golden/make_valley_digests.py [--case C] [--threads N] [--checkpoint DIR]
+    python tests/golden/make_full_run_digests.py [--case C] [--threads N] [--checkpoint DIR]
 --case roofline1024 runs the roofline generator at 1024 x 1024 (fractal
 terrain at 2 m, 10% NODATA, 5000 QFIX, free edges) for its full 3600 s
 instead, into tests/golden/roofline1024_fp64.json; --case memberK one

@@ -1,4 +1,4 @@
-"""The hand-over of tests/golden/make_valley_digests.py --from-fixture: an oracle
+"""The hand-over of tests/golden/make_full_run_digests.py --from-fixture: an oracle
 run resumed from a partial fixture and a state whose digests equal its last
 checkpoint continues exactly as the run that never stopped (same state, dt
 log, clock and mass reports); a state that differs in one bit is refused."""
@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def _digests():
     spec = importlib.util.spec_from_file_location(
-        "make_valley_digests", os.path.join(HERE, "golden", "make_valley_digests.py"))
+        "make_full_run_digests", os.path.join(HERE, "golden", "make_full_run_digests.py"))
     m = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(m)
     return m

@@ -121,7 +121,7 @@ def test_full_run_fixtures_are_consistent():
     import sys
 
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
-    import make_valley_digests as mk
+    import make_full_run_digests as mk
     paths = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                           "golden", "*_fp64.json")))
     cases = {os.path.basename(p)[: -len("_fp64.json")] for p in paths}

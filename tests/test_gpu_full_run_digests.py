@@ -4,7 +4,7 @@ valley at 5 m, 2000 QVAR hydrographs, rain over 0-1800 s, HFIX 2 m west
 edge, FREE N/E/S; 7200 s) and the roofline generator at 1024 x 1024
 (fractal terrain at 2 m, 10% NODATA, 5000 QFIX, FREE edges; 3600 s).  The
 fixtures (tests/golden/<case>_fp64.json, written by
-tests/golden/make_valley_digests.py from the oracle) keep digests of H, Qx
+tests/golden/make_full_run_digests.py from the oracle) keep digests of H, Qx
 and Qy every 5000 steps and at the end of the run, the digest of the dt log,
 and the mass reports.  The fused path must match every digest, so its
 state, fluxes and step sizes equal the oracle's bitwise over the whole run.
