@@ -30,6 +30,9 @@ def _faces_close(a, b, tol):
 @pytest.mark.parametrize("seed,nx,ny,rows", [
     (1, 37, 29, 8), (2, 64, 64, 16), (3, 130, 70, 32), (4, 257, 19, 5), (5, 300, 300, 128),
     (6, 1100, 40, 16), (7, 256, 48, 8), (8, 512, 20, 7),
+    # extreme aspects: 128 strips of three rows, one column pair down 4096
+    # rows, and one-row tiles down a three-column channel
+    (9, 16384, 3, 2), (10, 2, 4096, 64), (11, 3, 3000, 1),
 ])
 def test_fp64_matches_oracle_small(gpu, seed, nx, ny, rows):
     scn = _small(seed, nx, ny, sim_time=200.0)
